@@ -1,0 +1,97 @@
+// lookback.hpp -- tile status words and the wave-wide decoupled look-back
+// used by the single-pass scan, segmented scan/reduce and pack kernels.
+//
+// Each tile publishes its aggregate (flag kAggregate) as soon as it knows it,
+// and its inclusive prefix (flag kPrefix) once that is known.  A 32-bit value
+// and its flag share one 64-bit word; a 64-bit value is split over two words
+// that each carry the flag, and a reader accepts the pair only when both
+// flags agree.  Flags only move forward (invalid -> aggregate -> prefix), so
+// a reader that sees disagreeing flags simply reads again.
+#pragma once
+
+#include "cvl_hip.hpp"
+
+namespace cvlhip {
+
+enum : uint32_t { kInvalid = 0, kAggregate = 1, kPrefix = 2 };
+
+template <class T, int WORDS = (int)(sizeof(T) / 4)>
+struct TileStatus;
+
+template <class T>
+struct TileStatus<T, 1> {
+    static constexpr int kWords = 1;
+    __device__ static void publish(uint64_t *st, int64_t t, uint32_t flag, T v) {
+        st_status(st + t, ((uint64_t)flag << 32) | __builtin_bit_cast(uint32_t, v));
+    }
+    __device__ static uint32_t read(const uint64_t *st, int64_t t, T &v) {
+        uint64_t w = ld_status(st + t);
+        v = __builtin_bit_cast(T, (uint32_t)w);
+        return (uint32_t)(w >> 32);
+    }
+};
+
+template <class T>
+struct TileStatus<T, 2> {
+    static constexpr int kWords = 2;
+    __device__ static void publish(uint64_t *st, int64_t t, uint32_t flag, T v) {
+        uint64_t u = __builtin_bit_cast(uint64_t, v);
+        uint64_t f = (uint64_t)flag << 32;
+        st_status(st + 2 * t, f | (uint32_t)u);
+        st_status(st + 2 * t + 1, f | (uint32_t)(u >> 32));
+    }
+    __device__ static uint32_t read(const uint64_t *st, int64_t t, T &v) {
+        uint64_t w0 = ld_status(st + 2 * t);
+        uint64_t w1 = ld_status(st + 2 * t + 1);
+        uint32_t f0 = (uint32_t)(w0 >> 32), f1 = (uint32_t)(w1 >> 32);
+        if (f0 != f1) return kInvalid;
+        v = __builtin_bit_cast(T, ((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull));
+        return f0;
+    }
+};
+
+// Polls before a look-back gives up.  A valid launch never gets near it (every
+// tile publishes before it looks back, and tiles start in order); it only
+// guarantees that a corrupted status array cannot hang the GPU.  Giving up
+// raises bit 1 of the device error word.
+constexpr int kSpinLimit = 1 << 22;
+
+// Exclusive prefix of tile `tile`, computed by one whole wave.  Lane l
+// inspects tile (pos - l); the nearest tile holding a full prefix ends the
+// walk.  Values are combined in tile order (earliest operand first), so
+// order-sensitive operators (double max/min with signed zeros) agree with a
+// left fold.  Tiles before tile 0 read as a prefix equal to the neutral.
+template <class Op>
+__device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile, Op op,
+                                   unsigned *__restrict__ err) {
+    using T = typename Op::T;
+    using St = TileStatus<T>;
+    const int lane = lane_id();
+    T acc = Op::neutral();
+    int64_t pos = tile - 1;
+    for (;;) {
+        const int64_t j = pos - lane;
+        T v = Op::neutral();
+        uint32_t f = j >= 0 ? kInvalid : kPrefix;
+        for (int spins = 0;; spins++) {
+            if (f == kInvalid) f = St::read(st, j, v);
+            if (ballot(f == kInvalid) == 0) break;
+            if (spins >= kSpinLimit) {
+                if (f == kInvalid) { f = kPrefix; v = Op::neutral(); }
+                if (lane == 0) atomicOr(err, 2u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t pm = ballot(f == kPrefix);
+        const int L = pm ? (int)__builtin_ctzll(pm) : kWave;
+        T c = lane <= L ? v : Op::neutral();
+        T r = shfl(c, kWave - 1 - lane);          // lane 0 now holds the earliest tile
+        acc = op(wave_reduce(r, op), acc);
+        if (pm) break;
+        pos -= kWave;
+    }
+    return acc;
+}
+
+}  // namespace cvlhip

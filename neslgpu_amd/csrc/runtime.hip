@@ -1,0 +1,226 @@
+// runtime.hip -- library state, error handling, and the CVL facility
+// functions: sizes, memory, moves, host<->device copies, timers, RNG seed.
+//
+// Facility semantics follow the serial CVL (ref nesl/cvl/serial/facilt.c);
+// memory is device memory: vec_p is a device pointer, one CVL unit is
+// 8 bytes (the int/float/double union of facilt.c:92-93), and alo_foz is a
+// hipMalloc arena the caller sub-allocates (ref nesl/vcode/vstack.c:156-160).
+#include "cvl_hip.hpp"
+#include "../../include/cvl.h"
+#include "../../include/cvl_hip_ext.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+namespace cvlhip {
+
+hipStream_t g_stream = nullptr;
+static int g_fatal = 1;
+static char g_errbuf[512];
+static int g_err = 0;
+static unsigned *g_dev_err = nullptr;
+
+void fatal(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "cvl-hip: ");
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    va_end(ap);
+    exit(1);
+}
+
+void usage_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_errbuf, sizeof g_errbuf, fmt, ap);
+    va_end(ap);
+    g_err = 1;
+    if (g_fatal) fatal("%s", g_errbuf);
+}
+
+void hip_fail(hipError_t e, const char *expr, const char *file, int line) {
+    snprintf(g_errbuf, sizeof g_errbuf, "%s failed: %s (%s:%d)", expr, hipGetErrorString(e), file, line);
+    g_err = 2;
+    if (g_fatal) fatal("%s", g_errbuf);
+}
+
+unsigned *device_error_word() {
+    if (!g_dev_err) {
+        HIPCHECK(hipMalloc(&g_dev_err, 256));
+        HIPCHECK(hipMemsetAsync(g_dev_err, 0, 256, g_stream));
+    }
+    return g_dev_err;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, int64_t count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = v;
+}
+
+void fill_u32(void *p, uint32_t v, int64_t count) {
+    if (count <= 0) return;
+    int g = grid_for(count, kBlock * 4);
+    if (g > 8192) g = 8192;
+    k_fill_u32<<<g, kBlock, 0, g_stream>>>((uint32_t *)p, v, count);
+    LAUNCHCHECK();
+}
+
+// Overlap-safe move of `bytes` bytes (ref facilt.c:161-183 walks forward
+// when dest < src and backward otherwise).  Each block copies a chunk that
+// does not overlap the chunks still to be read by later blocks in the same
+// launch, so large overlapping moves are split into launches of
+// non-overlapping pieces.
+__global__ void k_copy_bytes16(const uint4 *__restrict__ s, uint4 *__restrict__ d, int64_t n16) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) d[i] = s[i];
+}
+
+static void copy_nonoverlap(void *d, const void *s, int64_t bytes) {
+    if (bytes <= 0) return;
+    HIPCHECK(hipMemcpyAsync(d, s, (size_t)bytes, hipMemcpyDeviceToDevice, g_stream));
+}
+
+void move_units(void *d, const void *s, int64_t units) {
+    if (d == s || units <= 0) return;
+    int64_t bytes = units * kUnit;
+    char *dc = (char *)d;
+    const char *sc = (const char *)s;
+    int64_t dist = dc > sc ? dc - sc : sc - dc;
+    if (dist >= bytes) { copy_nonoverlap(d, s, bytes); return; }
+    // Overlapping: copy in pieces of `dist` bytes, in the direction that
+    // never overwrites bytes not yet read.  Pieces are stream-ordered.
+    if (dc < sc) {
+        for (int64_t off = 0; off < bytes; off += dist) {
+            int64_t len = bytes - off < dist ? bytes - off : dist;
+            copy_nonoverlap(dc + off, sc + off, len);
+        }
+    } else {
+        for (int64_t end = bytes; end > 0; end -= dist) {
+            int64_t len = end < dist ? end : dist;
+            copy_nonoverlap(dc + end - len, sc + end - len, len);
+        }
+    }
+}
+
+}  // namespace cvlhip
+
+using namespace cvlhip;
+
+// ------------------------------------------------------------------ extension API
+extern "C" {
+
+void cvlhip_set_stream(void *s) { g_stream = (hipStream_t)s; }
+void *cvlhip_get_stream(void) { return (void *)g_stream; }
+void cvlhip_set_error_mode(int fatal_on_error) { g_fatal = fatal_on_error; }
+
+int cvlhip_last_error(char *buf, int len) {
+    int e = g_err;
+    if (buf && len > 0) {
+        strncpy(buf, g_err ? g_errbuf : "", (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    g_err = 0;
+    return e;
+}
+
+unsigned cvlhip_check_device_errors(void) {
+    unsigned h[1] = {0};
+    unsigned *w = device_error_word();
+    HIPCHECK(hipMemcpyAsync(h, w, 4, hipMemcpyDeviceToHost, g_stream));
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    if (h[0]) HIPCHECK(hipMemsetAsync(w, 0, 4, g_stream));
+    return h[0];
+}
+
+void cvlhip_synchronize(void) { HIPCHECK(hipStreamSynchronize(g_stream)); }
+
+// ------------------------------------------------------------------ CVL facilities
+void CVL_init(char *fused_library) {
+    (void)fused_library;   // fused libraries are loaded by the interpreter
+    device_error_word();
+}
+
+// Sizes in CVL units (ref facilt.c:95-116).  The segment descriptor of this
+// backend is larger than the serial one: see segd.hip.
+int siz_fob(int length) { return units_for_bytes((int64_t)length * 4); }
+int siz_foz(int length) { return units_for_bytes((int64_t)length * 4); }
+int siz_fod(int length) { return units_for_bytes((int64_t)length * 8); }
+
+vec_p alo_foz(int size) {
+    void *p = nullptr;
+    if (size < 0) return nullptr;
+    if (hipMalloc(&p, (size_t)(size > 0 ? size : 1) * kUnit) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;   // the caller reports failure (facilt.c:149-153)
+    }
+    return p;
+}
+
+void fre_fov(vec_p pointer) {
+    if (pointer) HIPCHECK(hipFree(pointer));
+}
+
+void mov_fov(vec_p d, vec_p s, int size, vec_p scratch) {
+    (void)scratch;
+    move_units(d, s, size);
+}
+int mov_fov_scratch(int size) { (void)size; return 0; }
+
+vec_p add_fov(vec_p v, int a) { return (vec_p)((char *)v + (int64_t)a * kUnit); }
+
+int sub_fov(vec_p v1, vec_p v2) {
+    int64_t d = ((char *)v1 - (char *)v2) / kUnit;
+    return (int)(d < 0 ? -d : d);
+}
+
+cvl_bool eql_fov(vec_p v1, vec_p v2) { return v1 == v2; }
+
+int cmp_fov(vec_p v1, vec_p v2) { return (int)(((char *)v1 - (char *)v2) / kUnit); }
+
+// Host <-> device copies (ref facilt.c:241-270).  The CVL call returns with
+// the host buffer filled (v2c) or already consumed (c2v).
+#define CVLHIP_V2C(name, ctype)                                                   \
+    void name(ctype *d, vec_p s, int len, vec_p scratch) {                        \
+        (void)scratch;                                                            \
+        if (len <= 0) return;                                                     \
+        HIPCHECK(hipMemcpyAsync(d, s, sizeof(ctype) * (size_t)len, hipMemcpyDeviceToHost, g_stream)); \
+        HIPCHECK(hipStreamSynchronize(g_stream));                                 \
+    }                                                                             \
+    int name##_scratch(int len) { (void)len; return 0; }                          \
+    unsigned int name##_inplace(void) { return INPLACE_NONE; }
+#define CVLHIP_C2V(name, ctype)                                                   \
+    void name(vec_p d, ctype *s, int len, vec_p scratch) {                        \
+        (void)scratch;                                                            \
+        if (len <= 0) return;                                                     \
+        HIPCHECK(hipMemcpyAsync(d, s, sizeof(ctype) * (size_t)len, hipMemcpyHostToDevice, g_stream)); \
+        HIPCHECK(hipStreamSynchronize(g_stream));                                 \
+    }                                                                             \
+    int name##_scratch(int len) { (void)len; return 0; }                          \
+    unsigned int name##_inplace(void) { return INPLACE_NONE; }
+
+CVLHIP_V2C(v2c_fuz, int)
+CVLHIP_V2C(v2c_fub, cvl_bool)
+CVLHIP_V2C(v2c_fud, double)
+CVLHIP_C2V(c2v_fuz, int)
+CVLHIP_C2V(c2v_fub, cvl_bool)
+CVLHIP_C2V(c2v_fud, double)
+
+// Timers (ref facilt.c:35-85): wall clock, as the serial CVL on Linux.
+// Device work issued before the call is finished first, so that
+// START_TIMER/STOP_TIMER bracket completed GPU work.
+void tgt_fos(cvl_timer_t *t) {
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    clock_gettime(CLOCK_REALTIME, t);
+}
+
+double tdf_fos(cvl_timer_t *t1, cvl_timer_t *t2) {
+    return (double)(t1->tv_sec - t2->tv_sec) + 1.0e-9 * (double)(t1->tv_nsec - t2->tv_nsec);
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// segd.hpp -- the segment descriptor of the HIP backend.
+//
+// The serial CVL's descriptor is the vector of m segment lengths
+// (ref nesl/cvl/serial/facilt.c:106-123).  Here the descriptor *starts*
+// with those lengths (so len_fos and anything that reads a descriptor as
+// lengths keeps working) and continues with data derived once by mke_fov,
+// which the segmented kernels use to find segment boundaries in O(1):
+//
+//   lens[m]          segment lengths (int32)
+//   offs[m+1]        exclusive prefix of lens; offs[m] = n
+//   nz[m]            indices of the non-empty segments, in order
+//   wcnt[W+1]        number of segment heads before each bitmap word
+//   heads[W]         bit j set when a non-empty segment starts at element j
+//                    (W = ceil(n/32) words, plus padding)
+//
+// The segment of element k is nz[rank(k) - 1] where rank(k) counts the
+// heads at or before k: wcnt[k>>5] + popcount(heads[k>>5] & low bits).
+// Every section starts on a 256-byte boundary.
+#pragma once
+
+#include <stdint.h>
+
+namespace cvlhip {
+
+struct SegdLayout {
+    int64_t lens, offs, nz, wcnt, heads, bytes;   // byte offsets and total size
+    int64_t words;                                 // bitmap words
+
+    __host__ __device__ static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+    __host__ __device__ SegdLayout(int64_t n, int64_t m) {
+        words = (n + 31) / 32;
+        lens = 0;
+        offs = up256(lens + 4 * m);
+        nz = up256(offs + 4 * (m + 1));
+        wcnt = up256(nz + 4 * m);
+        heads = up256(wcnt + 4 * (words + 1));
+        bytes = up256(heads + 4 * (words + 8));
+    }
+};
+
+struct SegdView {
+    const int *lens;
+    const int *offs;
+    const int *nz;
+    const int *wcnt;
+    const uint32_t *heads;
+    int64_t n, m;
+
+    __host__ __device__ SegdView()
+        : lens(nullptr), offs(nullptr), nz(nullptr), wcnt(nullptr), heads(nullptr), n(0), m(0) {}
+    __host__ SegdView(const void *sd, int64_t n_, int64_t m_) : n(n_), m(m_) {
+        SegdLayout L(n_, m_);
+        const char *b = (const char *)sd;
+        lens = (const int *)(b + L.lens);
+        offs = (const int *)(b + L.offs);
+        nz = (const int *)(b + L.nz);
+        wcnt = (const int *)(b + L.wcnt);
+        heads = (const uint32_t *)(b + L.heads);
+    }
+
+    // Is element k the first element of a (non-empty) segment?
+    __device__ __forceinline__ bool is_head(int64_t k) const {
+        return (heads[k >> 5] >> (k & 31)) & 1u;
+    }
+    // Segment index of element k (0 <= k < n).
+    __device__ __forceinline__ int seg_of(int64_t k) const {
+        uint32_t w = heads[k >> 5];
+        uint32_t mask = (uint32_t)(((uint64_t)2 << (k & 31)) - 1);
+        int r = wcnt[k >> 5] + __popc(w & mask);
+        return nz[r - 1];
+    }
+};
+
+}  // namespace cvlhip
