@@ -1,0 +1,304 @@
+"""Benchmark: the CVL primitive sweep at 2^28 int32 on one MI355X.
+
+One step = one pass of the sweep, each primitive called once through the
+C-ABI of libcvl_hip.so on inputs already resident in HBM:
+    add_suz   unsegmented exclusive +-scan, 2^28 int32           8 B/elem
+    max_ruz   unsegmented max-reduce, 2^28 int32                 4 B/elem
+    add_sez   segmented +-scan, geometric lengths (mean 256)     8 + 4m/n B/elem
+    pk2_luz   pack, Bernoulli(0.5) flags                         4 + 4 + 4*kept/n B/elem
+    smp_puz   permute by a random permutation                    12 B/elem
+    rku_luz   radix rank of 2^26 uniform 32-bit keys             8 B/key
+The byte counts are compulsory traffic (each input read once, each output
+written once, at CVL widths).  `value` is the whole sweep's compulsory bytes
+divided by its time (GB/s); per-primitive figures are in "primitives".
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+
+Prints one JSON line.  The reference serial CVL (compiled from its sources
+into oracle/_ref, else the C oracle) is timed on the host on a bounded
+sample of the same workload as `cpu_baseline`.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 0x4E45534C00000000
+CONFIG_ID = 2
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--log2n", type=int, default=28)
+    p.add_argument("--rank-log2n", type=int, default=26)
+    p.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
+    p.add_argument("--only", default="", help="comma list of primitives to run (profiling)")
+    return p.parse_args()
+
+
+def geometric_lengths(n: int, mean: int, seed: int) -> np.ndarray:
+    """Segment lengths >= 1 with a geometric distribution, summing to n."""
+    rng = np.random.default_rng(seed)
+    m_guess = n // mean + 1024
+    lens = rng.geometric(1.0 / mean, size=m_guess).astype(np.int64)
+    cs = np.cumsum(lens)
+    k = int(np.searchsorted(cs, n))
+    lens = lens[: k + 1]
+    lens[-1] -= int(cs[k] - n)
+    lens = lens[lens > 0]
+    assert int(lens.sum()) == n
+    return lens.astype(np.int32)
+
+
+class Sweep:
+    def __init__(self, log2n: int, rank_log2n: int):
+        import torch
+        from neslgpu_amd.cvl import Cvl
+        self.torch = torch
+        self.cv = Cvl()
+        self.lib = self.cv.lib
+        n = 1 << log2n
+        nr = 1 << rank_log2n
+        self.n, self.nr = n, nr
+        dev = self.cv.device
+        i32 = torch.int32
+        L = self.lib
+        # inputs: splitmix64 streams generated on the device
+        self.s = torch.empty(n, dtype=i32, device=dev)
+        L.cvlhip_fill_splitmix_u32(self.s, n, SEED + CONFIG_ID)
+        self.keys = torch.empty(nr, dtype=i32, device=dev)
+        L.cvlhip_fill_splitmix_u32(self.keys, nr, SEED + CONFIG_ID + 1)
+        u = torch.empty(n, dtype=i32, device=dev)
+        L.cvlhip_fill_splitmix_u32(u, n, SEED + CONFIG_ID + 2)
+        self.flags = (u < 0).to(i32)                       # top bit: Bernoulli(0.5)
+        del u
+        g = torch.Generator(device=dev)
+        g.manual_seed(SEED + CONFIG_ID + 3 & 0x7FFFFFFF)
+        self.perm = torch.randperm(n, device=dev, generator=g, dtype=torch.int64).to(i32)
+        self.lens = geometric_lengths(n, 256, SEED + CONFIG_ID + 4 & 0x7FFFFFFF)
+        self.m = int(self.lens.size)
+        self.segd = self.cv.segd(self.lens)
+        # outputs and scratch
+        self.d = torch.empty(n, dtype=i32, device=dev)
+        self.kept = int(self.flags.sum().item())
+        self.rank = torch.empty(nr, dtype=i32, device=dev)
+        # one scratch area serves every primitive: the largest of their
+        # *_scratch() requirements (CVL units of 8 bytes)
+        units = max(L.add_suz_scratch(n), L.max_ruz_scratch(n), L.add_sez_scratch(n, self.m),
+                    L.pk2_luz_scratch(n), L.smp_puz_scratch(n), L.rku_luz_scratch(nr), 1)
+        self.scr = torch.empty(units * 8 + 4096, dtype=torch.uint8, device=dev)
+        self.max_result = None
+        torch.cuda.synchronize()
+
+    # compulsory bytes of each primitive
+    def bytes(self) -> dict:
+        n, nr, m = self.n, self.nr, self.m
+        return {
+            "add_suz": 8 * n,
+            "max_ruz": 4 * n,
+            "add_sez": 8 * n + 4 * m,
+            "pk2_luz": 8 * n + 4 * self.kept,
+            "smp_puz": 12 * n,
+            "rku_luz": 8 * nr,
+        }
+
+    def run(self, name: str):
+        L, n = self.lib, self.n
+        if name == "add_suz":
+            L.add_suz(self.d, self.s, n, self.scr)
+        elif name == "max_ruz":
+            self.max_result = L.max_ruz(self.s, n, self.scr)
+        elif name == "add_sez":
+            L.add_sez(self.d, self.s, self.segd, n, self.m, self.scr)
+        elif name == "pk2_luz":
+            L.pk2_luz(self.d, self.s, self.flags, n, self.kept, self.scr)
+        elif name == "smp_puz":
+            L.smp_puz(self.d, self.s, self.perm, n, self.scr)
+        elif name == "rku_luz":
+            L.rku_luz(self.rank, self.keys, self.nr, self.scr)
+
+    def step(self, names):
+        for k in names:
+            self.run(k)
+
+
+def time_gpu(fn, reps: int, torch) -> float:
+    """Average milliseconds of fn() over reps, HIP events on the current stream."""
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+# ------------------------------------------------------------------ host baseline
+def cpu_baseline(budget_s: float = 20.0) -> dict:
+    """The serial CVL on this host, single thread, on a bounded sample of the
+    sweep: 2^24 elements for the n-sized primitives (64 MiB per vector, far
+    beyond the host caches), 2^22 keys for rank."""
+    from oracle import oracle as O
+    from oracle import refcvl
+    kind = "reference" if refcvl.available() else "port"
+    n = 1 << 24
+    nr = 1 << 22
+    s = O.splitmix_u32(n, SEED + CONFIG_ID).view(np.int32)
+    keys = O.splitmix_u32(nr, SEED + CONFIG_ID + 1).view(np.int32)
+    flags = (O.splitmix_u32(n, SEED + CONFIG_ID + 2) >> 31).astype(np.int32)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(n).astype(np.int32)
+    lens = geometric_lengths(n, 256, SEED + CONFIG_ID + 4 & 0x7FFFFFFF)
+    m = lens.size
+    kept = int(flags.sum())
+    d = np.empty(n, np.int32)
+    rk = np.empty(nr, np.int32)
+    scr = np.zeros(4 * nr + 16, np.float64)
+    byts = {"add_suz": 8 * n, "max_ruz": 4 * n, "add_sez": 8 * n + 4 * m, "pk2_luz": 8 * n + 4 * kept,
+            "smp_puz": 12 * n, "rku_luz": 8 * nr}
+    times = {}
+    t_start = time.perf_counter()
+    if kind == "reference":
+        call = refcvl.call
+        ops = {
+            "add_suz": lambda: call("add_suz", d, s, n, scr),
+            "max_ruz": lambda: call("max_ruz", s, n, scr),
+            "add_sez": lambda: call("add_sez", d, s, lens, n, m, scr),
+            "pk2_luz": lambda: call("pk2_luz", d, s, flags, n, kept, scr),
+            "smp_puz": lambda: call("smp_puz", d, s, perm, n, scr),
+            "rku_luz": lambda: call("rku_luz", rk, keys, nr, scr),
+        }
+    else:
+        ops = {
+            "add_suz": lambda: O.scan("add", "z", s),
+            "max_ruz": lambda: O.reduce("max", "z", s),
+            "add_sez": lambda: O.segscan("add", "z", s, lens),
+            "pk2_luz": lambda: O.pk2("z", s, flags),
+            "smp_puz": lambda: O.smp_pu("z", s, perm),
+            "rku_luz": lambda: O.rank(keys, True),
+        }
+    for k, f in ops.items():
+        f()                                  # warm caches and page in
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            f()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 12 or reps >= 5:
+                break
+        times[k] = el / reps
+    total_bytes = sum(byts.values())
+    total_time = sum(times.values())
+    return {
+        "value": round(total_bytes / total_time / 1e9, 3),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "same sweep at 2^24 elements (rank: 2^22 keys), splitmix64 inputs, serial CVL -O2 "
+                  "-fwrapv, mean of up to 5 timed runs per primitive after one warm-up run",
+        "per_primitive_gbs": {k: round(byts[k] / times[k] / 1e9, 3) for k in times},
+        "host_cpu": _cpu_model(),
+        "host_nproc": os.cpu_count(),
+        "wall_s": round(time.perf_counter() - t_start, 2),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    sw = Sweep(a.log2n, a.rank_log2n)
+    names = ["add_suz", "max_ruz", "add_sez", "pk2_luz", "smp_puz", "rku_luz"]
+    if a.only:
+        names = [x for x in names if x in a.only.split(",")]
+    byts = sw.bytes()
+    for _ in range(a.warmup):
+        sw.step(names)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = time_gpu(lambda: sw.step(names), a.steps, torch)
+    wall = (time.perf_counter() - t0) / a.steps * 1e3
+    total_bytes = sum(byts[k] for k in names)
+    prims = {}
+    for k in names:
+        t = time_gpu(lambda k=k: sw.run(k), a.steps, torch)
+        gbs = byts[k] / t / 1e6
+        prims[k] = {"ms": round(t, 4), "GB/s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                    "bytes": byts[k]}
+    dom = max(prims, key=lambda k: prims[k]["ms"])
+    kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce (MaxZ)",
+                 "add_sez": "k_scan (AddZ, segmented)", "pk2_luz": "k_pack<int>",
+                 "smp_puz": "k_scatter<int>", "rku_luz": "k_rank_pass<uint32_t> (4 passes)"}
+    roof = {
+        "bound": "hbm",
+        "kernel": kernel_of[dom],
+        "primitive": dom,
+        "achieved": prims[dom]["GB/s"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": prims[dom]["frac_of_hbm_peak"],
+        "traffic": None,
+    }
+    value = total_bytes / ms / 1e6
+    out = {
+        "metric": "CVL primitive GB/s vs HBM peak; quickhull/SpMV time, learned vs fixed dispatch",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "wall_ms_per_step": round(wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (splitmix64 keys/values, geometric(mean 256) segment lengths, "
+                "Bernoulli(0.5) flags, uniform random permutation)",
+        "config": {"workload": f"CVL primitive sweep: 2^{a.log2n} int32 (add_suz, max_ruz, add_sez, "
+                               f"pk2_luz, smp_puz) + rku_luz on 2^{a.rank_log2n} keys",
+                   "n": sw.n, "segments": sw.m, "rank_keys": sw.nr, "primitives_run": names},
+        "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+        "primitives": prims,
+        "roofline": roof,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,10 @@ int cvlhip_last_error(char *buf, int len);
  * last call (the access was skipped); clears the flag. */
 unsigned cvlhip_check_device_errors(void);
 void cvlhip_synchronize(void);
+/* Grid of the tile kernels (scan, pack): cap > 0 keeps at most `cap`
+ * resident blocks per compute unit, each looping over tiles; 0 = as many
+ * resident blocks as the occupancy allows; -1 = one block per tile. */
+void cvlhip_set_blocks_per_cu(int cap);
 
 /* ---- random numbers -------------------------------------------------- */
 /* The 48-bit lrand48 state behind RAND (rnd_wuz).  It starts at 0, as an

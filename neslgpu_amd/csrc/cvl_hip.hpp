@@ -144,6 +144,13 @@ struct MaxD {
     __host__ __device__ T operator()(T a, T b) const { return a > b ? a : b; }
 };
 
+struct AddU {
+    using T = unsigned;
+    __host__ __device__ static T neutral() { return 0u; }
+    __host__ __device__ static T cvl_id() { return 0u; }
+    __host__ __device__ T operator()(T a, T b) const { return a + b; }
+};
+
 // Number of true flags: cvl_bool values normalised to 0/1, then added
 // (ref nesl/cvl/serial/library.c:66-75, 114-133 count `if (flag)`).
 struct CountB {
@@ -250,6 +257,12 @@ __device__ __forceinline__ uint64_t ld_status(const uint64_t *p) {
 __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Number of blocks of `kernel` (kBlock threads) that fit on the device at
+// once, capped at `work` and at the blocks-per-CU limit set by
+// cvlhip_set_blocks_per_cu (0 = as many as fit).  Persistent kernels use it
+// as their grid.
+int64_t resident_grid(const void *kernel, int64_t work);
 
 // ------------------------------------------------------------------ fills
 void fill_u32(void *p, uint32_t v, int64_t count);   // stream-ordered fill kernel

@@ -56,9 +56,17 @@ struct TileStatus<T, 2> {
 // raises bit 1 of the device error word.
 constexpr int kSpinLimit = 1 << 22;
 
-// Exclusive prefix of tile `tile`, computed by one whole wave.  Lane l
-// inspects tile (pos - l); the nearest tile holding a full prefix ends the
-// walk.  Values are combined in tile order (earliest operand first), so
+// Tiles inspected by each lane per look-back round (the window is 64*K).
+#ifndef CVLHIP_LOOKBACK_K
+#define CVLHIP_LOOKBACK_K 1
+#endif
+constexpr int kLookK = CVLHIP_LOOKBACK_K;
+
+// Exclusive prefix of tile `tile`, computed by one whole wave.  Each round
+// inspects the 64*K tiles before `pos`: lane l reads tiles
+// pos - l*K - j (j = 0..K-1, all loads in flight together).  The nearest
+// tile holding a full prefix ends the walk; otherwise the window slides back.
+// Values are combined in tile order (earliest operand first), so
 // order-sensitive operators (double max/min with signed zeros) agree with a
 // left fold.  Tiles before tile 0 read as a prefix equal to the neutral.
 template <class Op>
@@ -66,30 +74,54 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
                                    unsigned *__restrict__ err) {
     using T = typename Op::T;
     using St = TileStatus<T>;
+    constexpr int K = kLookK;
     const int lane = lane_id();
     T acc = Op::neutral();
     int64_t pos = tile - 1;
     for (;;) {
-        const int64_t j = pos - lane;
-        T v = Op::neutral();
-        uint32_t f = j >= 0 ? kInvalid : kPrefix;
+        T v[K];
+        uint32_t f[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int64_t t = pos - (int64_t)lane * K - j;
+            v[j] = Op::neutral();
+            f[j] = t >= 0 ? kInvalid : kPrefix;
+        }
         for (int spins = 0;; spins++) {
-            if (f == kInvalid) f = St::read(st, j, v);
-            if (ballot(f == kInvalid) == 0) break;
+            bool waiting = false;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int64_t t = pos - (int64_t)lane * K - j;
+                if (f[j] == kInvalid) f[j] = St::read(st, t, v[j]);
+                waiting |= f[j] == kInvalid;
+            }
+            if (ballot(waiting) == 0) break;
             if (spins >= kSpinLimit) {
-                if (f == kInvalid) { f = kPrefix; v = Op::neutral(); }
+#pragma unroll
+                for (int j = 0; j < K; j++)
+                    if (f[j] == kInvalid) { f[j] = kPrefix; v[j] = Op::neutral(); }
                 if (lane == 0) atomicOr(err, 2u);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        const uint64_t pm = ballot(f == kPrefix);
+        // nearest prefix within this lane's K tiles
+        int fj = K;
+#pragma unroll
+        for (int j = K - 1; j >= 0; j--)
+            if (f[j] == kPrefix) fj = j;
+        const uint64_t pm = ballot(fj < K);
         const int L = pm ? (int)__builtin_ctzll(pm) : kWave;
-        T c = lane <= L ? v : Op::neutral();
-        T r = shfl(c, kWave - 1 - lane);          // lane 0 now holds the earliest tile
+        // lane-local fold, earliest tile first, over the tiles this lane contributes
+        const int lim = lane < L ? K - 1 : (lane == L ? fj : -1);
+        T c = Op::neutral();
+#pragma unroll
+        for (int j = K - 1; j >= 0; j--)
+            if (j <= lim) c = op(c, v[j]);
+        T r = shfl(c, kWave - 1 - lane);          // lane 0 now holds the earliest tiles
         acc = op(wave_reduce(r, op), acc);
         if (pm) break;
-        pos -= kWave;
+        pos -= (int64_t)kWave * K;
     }
     return acc;
 }

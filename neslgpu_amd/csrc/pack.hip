@@ -28,7 +28,10 @@ template <> struct VecN<double> {
     typedef int ft __attribute__((ext_vector_type(2)));
 };
 
-constexpr int kPackRows = 4;
+#ifndef CVLHIP_PACK_ROWS
+#define CVLHIP_PACK_ROWS 16
+#endif
+constexpr int kPackRows = CVLHIP_PACK_ROWS;
 
 template <class T, int R>
 struct PackShape {

@@ -1,0 +1,264 @@
+// partition.hip -- permute by a permutation without per-element scattered
+// stores (the engine behind smp_pu*/smp_pe*, ref nesl/cvl/serial/
+// vprims.c:356-405, and the final inversion of the radix rank).
+//
+// On MI355X a store whose 64 lanes hit 64 different cache lines costs ~64
+// line transactions whatever the locality, which caps a naive scatter at
+// ~45 G elements/s.  Here every store is coalesced:
+//
+//   partition passes  Destination space is cut into blocks of kBlk elements.
+//                     Because the indices form a permutation, destination
+//                     block c receives exactly its size, so pairs (index,
+//                     value) can be bucketed by block without a histogram
+//                     pass: a tile counts its elements per bucket in LDS,
+//                     claims a slot range per bucket with one atomicAdd,
+//                     stages the pairs bucket by bucket in LDS and writes
+//                     each bucket's run contiguously.  One pass splits a
+//                     region into <= 256 children; two passes take 2^28
+//                     elements to 8192-element blocks.
+//   block scatter     One workgroup per destination block scatters its
+//                     pairs into LDS and writes the block out in order.
+//
+// Pairs whose index is out of range, or that overflow their bucket (the
+// indices were not a permutation), are dropped and raise the device error
+// word; destination positions nobody wrote keep their old contents.
+#include "cvl_hip.hpp"
+#include "segd.hpp"
+#include "partition.hpp"
+#include "../../include/cvl.h"
+
+namespace cvlhip {
+
+constexpr int kBlkLog = 13;
+constexpr int64_t kBlk = 1 << kBlkLog;         // final destination block (LDS-resident)
+constexpr int kPartThreads = 512;
+constexpr int kPartItems = 16;
+constexpr int kPartTile = kPartThreads * kPartItems;   // 8192 pairs per tile
+constexpr int kMaxFan = 256;                   // children per region per pass
+
+// Where the destination index of input element k comes from on the first
+// pass: p[k] (plain permute) or offs[seg(k)] + p[k] (segmented permute).
+struct IndexSource {
+    const int *p;
+    SegdView seg;
+    bool segmented;
+    __device__ __forceinline__ int64_t at(int64_t k) const {
+        int64_t q = p[k];
+        if (segmented) q += seg.offs[seg.seg_of(k)];
+        return q;
+    }
+};
+
+// One partition pass.  Region r = [r*psize, min((r+1)*psize, n)) of the
+// input is split into children of csize elements (csize divides psize,
+// psize/csize <= 256).  Block b handles tile (b % tiles_per_region) of
+// region (b / tiles_per_region).  Child c's pairs land in
+// [c*csize, c*csize + cnt[c]) of the output.
+template <class T, bool FIRST>
+__global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, const int *__restrict__ in_idx,
+                                                            const T *__restrict__ in_val, int *__restrict__ out_idx,
+                                                            T *__restrict__ out_val, int64_t n, int plog, int clog,
+                                                            int64_t tiles_per_region,
+                                                            const unsigned *__restrict__ prev_cnt,
+                                                            unsigned *__restrict__ cnt, unsigned *__restrict__ err) {
+    __shared__ unsigned s_hist[kMaxFan];
+    __shared__ unsigned s_start[kMaxFan];
+    __shared__ unsigned s_gofs[kMaxFan];
+    __shared__ int s_idx[kPartTile];
+    __shared__ T s_val[kPartTile];
+    __shared__ unsigned char s_dig[kPartTile];
+
+    const int tid = threadIdx.x;
+    const int64_t region = blockIdx.x / tiles_per_region;
+    const int64_t chunk = blockIdx.x % tiles_per_region;
+    const int64_t rlo = region << plog;
+    int64_t rhi = (rlo + ((int64_t)1 << plog)) < n ? rlo + ((int64_t)1 << plog) : n;
+    if (!FIRST && rlo + (int64_t)prev_cnt[region] < rhi) rhi = rlo + prev_cnt[region];   // pairs that landed
+    const int64_t lo = rlo + chunk * kPartTile;
+    const int64_t first_child = rlo >> clog;
+
+    for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
+    __syncthreads();
+
+    int q[kPartItems];
+    T v[kPartItems];
+    unsigned rk[kPartItems];
+    int dg[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; j++) {
+        const int64_t e = lo + (int64_t)j * kPartThreads + tid;
+        dg[j] = -1;
+        if (e < rhi) {
+            int64_t qq;
+            if constexpr (FIRST) {
+                qq = src.at(e);
+                v[j] = in_val[e];
+            } else {
+                qq = in_idx[e];
+                v[j] = in_val[e];
+            }
+            const int64_t child = qq >> clog;
+            const int64_t d = child - first_child;
+            if (qq >= 0 && qq < n && d >= 0 && d < kMaxFan && (qq >> plog) == region) {
+                q[j] = (int)qq;
+                dg[j] = (int)d;
+                rk[j] = atomicAdd(&s_hist[d], 1u);
+            } else {
+                atomicOr(err, 1u);
+            }
+        }
+    }
+    __syncthreads();
+    // Exclusive scan of the 256 counts (one wave) and one global claim per
+    // non-empty bucket.
+    if (tid < kWave) {
+        unsigned c[kMaxFan / kWave], tot = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxFan / kWave; i++) { c[i] = s_hist[tid * (kMaxFan / kWave) + i]; tot += c[i]; }
+        unsigned inc = wave_inclusive_scan(tot, AddU());
+        unsigned run = inc - tot;
+#pragma unroll
+        for (int i = 0; i < kMaxFan / kWave; i++) {
+            const int d = tid * (kMaxFan / kWave) + i;
+            s_start[d] = run;
+            run += c[i];
+            s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPartItems; j++) {
+        if (dg[j] >= 0) {
+            const unsigned at = s_start[dg[j]] + rk[j];
+            s_idx[at] = q[j];
+            s_val[at] = v[j];
+            s_dig[at] = (unsigned char)dg[j];
+        }
+    }
+    __syncthreads();
+    const int64_t total = (int64_t)s_start[kMaxFan - 1] + s_hist[kMaxFan - 1];
+    const int64_t csize = (int64_t)1 << clog;
+    for (int64_t i = tid; i < total; i += kPartThreads) {
+        const int d = s_dig[i];
+        const int64_t child = first_child + d;
+        const int64_t slot = (int64_t)s_gofs[d] + (i - s_start[d]);
+        const int64_t cend = (child + 1) * csize < n ? (child + 1) * csize : n;
+        const int64_t pos = child * csize + slot;
+        if (pos < cend) {
+            out_idx[pos] = s_idx[i];
+            out_val[pos] = s_val[i];
+        } else {
+            atomicOr(err, 1u);   // more pairs than the block holds: not a permutation
+        }
+    }
+}
+
+// Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
+// first cnt[c] entries of the same range of (idx, val).
+template <class T, bool DIRECT>
+__global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src, const int *__restrict__ in_idx,
+                                                                const T *__restrict__ in_val, T *__restrict__ d,
+                                                                int64_t n, const unsigned *__restrict__ cnt,
+                                                                unsigned *__restrict__ err) {
+    __shared__ T s_out[kBlk];
+    __shared__ unsigned s_mask[kBlk / 32];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kBlk;
+    const int64_t end = base + kBlk < n ? base + kBlk : n;
+    for (int i = tid; i < kBlk / 32; i += kPartThreads) s_mask[i] = 0;
+    __syncthreads();
+    int64_t count = end - base;
+    if (!DIRECT && (int64_t)cnt[blockIdx.x] < count) count = cnt[blockIdx.x];
+    for (int64_t i = tid; i < count; i += kPartThreads) {
+        int64_t q;
+        T v;
+        if constexpr (DIRECT) {
+            q = src.at(base + i);
+            v = in_val[base + i];
+        } else {
+            q = in_idx[base + i];
+            v = in_val[base + i];
+        }
+        const int64_t o = q - base;
+        if (o >= 0 && o < end - base) {
+            s_out[o] = v;
+            atomicOr(&s_mask[o >> 5], 1u << (o & 31));
+        } else {
+            atomicOr(err, 1u);
+        }
+    }
+    __syncthreads();
+    for (int64_t o = tid; o < end - base; o += kPartThreads)
+        if ((s_mask[o >> 5] >> (o & 31)) & 1u) d[base + o] = s_out[o];
+}
+
+static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+// Scratch: two (idx, val) buffers of n pairs and one counter array per
+// partition pass (at most 3 passes for n < 2^31).
+int64_t partition_permute_scratch_bytes(int64_t n, int width) {
+    const int64_t pair = up256(4 * n) + up256((int64_t)width * n);
+    const int64_t counters = 3 * up256(4 * ((n >> kBlkLog) + 2));
+    return 2 * pair + counters + 256;
+}
+
+template <class T>
+void partition_permute(void *d, const void *s, const int *p, const void *sd, int64_t n, int64_t m, void *scratch) {
+    if (n <= 0) return;
+    IndexSource src{p, SegdView(sd, n, sd ? m : 0), sd != nullptr && m > 1};
+    unsigned *err = device_error_word();
+    // Region sizes per pass: from ceil-pow2(n) down to kBlk, at most 8 bits
+    // per pass.
+    int nlog = 0;
+    while (((int64_t)1 << nlog) < n) nlog++;
+    if (nlog <= kBlkLog) {
+        k_block_scatter<T, true><<<1, kPartThreads, 0, g_stream>>>(src, nullptr, (const T *)s, (T *)d, n, nullptr, err);
+        LAUNCHCHECK();
+        return;
+    }
+    if (!scratch) usage_error("permute called without scratch");
+    auto up = up256;
+    char *b = (char *)scratch;
+    int *idx[2];
+    T *val[2];
+    idx[0] = (int *)b; b += up(4 * n);
+    val[0] = (T *)b; b += up((int64_t)sizeof(T) * n);
+    idx[1] = (int *)b; b += up(4 * n);
+    val[1] = (T *)b; b += up((int64_t)sizeof(T) * n);
+    const int levels = (nlog - kBlkLog + 7) / 8;
+    int plog = nlog;
+    int cur = -1;
+    unsigned *cnt = nullptr, *prev = nullptr;
+    for (int l = 0; l < levels; l++) {
+        const int bits = (plog - kBlkLog) - 8 * (levels - 1 - l);   // the first pass takes the odd bits
+        const int clog = plog - bits;
+        const int64_t nchild = ((n - 1) >> clog) + 1;
+        prev = cnt;
+        cnt = (unsigned *)b;
+        b += up(4 * (nchild + 1));
+        fill_u32(cnt, 0, nchild + 1);
+        const int64_t regions = ((n - 1) >> plog) + 1;
+        const int64_t rsize = (int64_t)1 << plog;
+        const int64_t tpr = (rsize < n ? rsize + kPartTile - 1 : n + kPartTile - 1) / kPartTile;
+        const int64_t grid = regions * tpr;
+        const int out = cur == 0 ? 1 : 0;
+        if (cur < 0)
+            k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                src, nullptr, (const T *)s, idx[out], val[out], n, plog, clog, tpr, nullptr, cnt, err);
+        else
+            k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                src, idx[cur], val[cur], idx[out], val[out], n, plog, clog, tpr, prev, cnt, err);
+        LAUNCHCHECK();
+        cur = out;
+        plog = clog;
+    }
+    const int64_t blocks = ((n - 1) >> kBlkLog) + 1;
+    k_block_scatter<T, false><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(src, idx[cur], val[cur], (T *)d, n,
+                                                                              cnt, err);
+    LAUNCHCHECK();
+}
+
+template void partition_permute<int>(void *, const void *, const int *, const void *, int64_t, int64_t, void *);
+template void partition_permute<double>(void *, const void *, const int *, const void *, int64_t, int64_t, void *);
+
+}  // namespace cvlhip

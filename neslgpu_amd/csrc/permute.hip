@@ -19,6 +19,7 @@
 // source position aimed at it (atomicMax), then only that position writes.
 #include "cvl_hip.hpp"
 #include "segd.hpp"
+#include "partition.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
@@ -142,16 +143,14 @@ static int lww_units(int n_d) { return units_for_bytes(4 * (int64_t)(n_d > 0 ? n
 
 #define PERMS(TY, T, W)                                                                                   \
     void smp_pu##TY(vec_p d, vec_p s, vec_p i, int len, vec_p scratch) {                                   \
-        (void)scratch;                                                                                    \
-        scatter<T>(d, s, i, nullptr, len, len, no_seg());                                                 \
+        partition_permute<T>(d, s, (const int *)i, nullptr, len, 0, scratch);                             \
     }                                                                                                     \
-    int smp_pu##TY##_scratch(int len) { (void)len; return 0; }                                            \
+    int smp_pu##TY##_scratch(int len) { return units_for_bytes(partition_permute_scratch_bytes(len, W)); } \
     unsigned int smp_pu##TY##_inplace(void) { return INPLACE_NONE; }                                      \
     void smp_pe##TY(vec_p d, vec_p s, vec_p i, vec_p sd, int n, int m, vec_p scratch) {                    \
-        (void)scratch;                                                                                    \
-        scatter<T>(d, s, i, nullptr, n, n, SEGB(sd, n, m, nullptr));                                      \
+        partition_permute<T>(d, s, (const int *)i, sd, n, m, scratch);                                    \
     }                                                                                                     \
-    int smp_pe##TY##_scratch(int n, int m) { (void)n; (void)m; return 0; }                                \
+    int smp_pe##TY##_scratch(int n, int m) { (void)m; return units_for_bytes(partition_permute_scratch_bytes(n, W)); } \
     unsigned int smp_pe##TY##_inplace(void) { return INPLACE_NONE; }                                      \
     void bck_pu##TY(vec_p d, vec_p s, vec_p i, int s_len, int d_len, vec_p scratch) {                     \
         (void)scratch;                                                                                    \

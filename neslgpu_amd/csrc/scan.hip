@@ -37,7 +37,10 @@ template <class Op> struct ordered { static constexpr bool value = false; };
 template <> struct ordered<MaxD> { static constexpr bool value = true; };
 template <> struct ordered<MinD> { static constexpr bool value = true; };
 
-constexpr int kScanRows = 4;   // rows per wave in one tile
+#ifndef CVLHIP_SCAN_ROWS
+#define CVLHIP_SCAN_ROWS 16
+#endif
+constexpr int kScanRows = CVLHIP_SCAN_ROWS;   // rows per wave in one tile
 
 // Load V elements starting at element `base` (neutral beyond n).
 template <class Op, int V>
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(const typename Op::T *__restric
     // Wave-local scan of the wave's R rows.
     T carry = Op::neutral();
     int cf = 0;
-    uint32_t needs = 0;
+    uint64_t needs = 0;     // bit r*V+k: element still needs the incoming prefix
+    static_assert(R * V <= 64, "needs mask holds one bit per element of a lane");
 #pragma unroll
     for (int r = 0; r < R; r++) {
         T a = Op::neutral();
@@ -180,10 +184,10 @@ __global__ __launch_bounds__(kBlock) void k_scan(const typename Op::T *__restric
             T y;
             if constexpr (!REDUCE) {
                 if (h) { y = Op::neutral(); p = x[r][k]; pf = 1; }
-                else { y = p; if (!pf) needs |= 1u << (r * V + k); p = op(p, x[r][k]); }
+                else { y = p; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
             } else {
                 if (h) { p = x[r][k]; pf = 1; }
-                else { p = op(p, x[r][k]); if (!pf) needs |= 1u << (r * V + k); }
+                else { p = op(p, x[r][k]); if (!pf) needs |= 1ull << (r * V + k); }
                 y = p;
             }
             x[r][k] = y;
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const typename Op::T *__restric
 #pragma unroll
         for (int k = 0; k < V; k++) {
             T v = x[r][k];
-            if ((needs >> (r * V + k)) & 1u) v = op(inc, v);
+            if ((needs >> (r * V + k)) & 1ull) v = op(inc, v);
             y[k] = op(cid, v);
         }
         if constexpr (!REDUCE) {
@@ -286,15 +290,19 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restr
     const int64_t whi = wlo + per_wave < hi ? wlo + per_wave : hi;
 
     T acc = Op::neutral();
-    for (int64_t rb = wlo; rb < whi; rb += S::ROW) {
-        T x[V];
-        const int64_t base = rb + (int64_t)lane * V;
-        load_lane<Op, V>(in, base, whi, aligned, x);
-        T a = x[0];
+    constexpr int U = 4;                 // rows in flight per lane
+    for (int64_t rb = wlo; rb < whi; rb += U * S::ROW) {
+        T x[U][V];
 #pragma unroll
-        for (int k = 1; k < V; k++) a = op(a, x[k]);
-        if constexpr (ordered<Op>::value) acc = op(acc, wave_reduce(a, op));
-        else acc = op(acc, a);
+        for (int u = 0; u < U; u++) load_lane<Op, V>(in, rb + (int64_t)u * S::ROW + (int64_t)lane * V, whi, aligned, x[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            T a = x[u][0];
+#pragma unroll
+            for (int k = 1; k < V; k++) a = op(a, x[u][k]);
+            if constexpr (ordered<Op>::value) acc = op(acc, wave_reduce(a, op));
+            else acc = op(acc, a);
+        }
     }
     if constexpr (!ordered<Op>::value) acc = wave_reduce(acc, op);
     if (lane == 0) s_w[w] = acc;
