@@ -25,27 +25,41 @@
 #include "cvl_hip.hpp"
 #include "segd.hpp"
 #include "partition.hpp"
+#include "colscan.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
 
 constexpr int kBlkLog = 13;
 constexpr int64_t kBlk = 1 << kBlkLog;         // final destination block (LDS-resident)
+#ifndef CVLHIP_PART_ITEMS
+#define CVLHIP_PART_ITEMS 8
+#endif
 constexpr int kPartThreads = 512;
-constexpr int kPartItems = 16;
-constexpr int kPartTile = kPartThreads * kPartItems;   // 8192 pairs per tile
+constexpr int kPartItems = CVLHIP_PART_ITEMS;
+constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
 constexpr int kMaxFan = 256;                   // children per region per pass
 
-// Where the destination index of input element k comes from on the first
-// pass: p[k] (plain permute) or offs[seg(k)] + p[k] (segmented permute).
+// Where the destination index and the value of input element k come from
+// on the first pass: the index is p[k] (plain permute) or offs[seg(k)] + p[k]
+// (segmented permute); the value is s[k], or -- for inverting a sort order
+// -- k itself, optionally relative to the start of k's segment in `vseg`.
 struct IndexSource {
     const int *p;
     SegdView seg;
     bool segmented;
+    int vmode;        // 0: s[k]; 1: k; 2: k - offs[seg_of(k)] in vseg
+    SegdView vseg;
     __device__ __forceinline__ int64_t at(int64_t k) const {
         int64_t q = p[k];
         if (segmented) q += seg.offs[seg.seg_of(k)];
         return q;
+    }
+    template <class T>
+    __device__ __forceinline__ T value(const T *__restrict__ s, int64_t k) const {
+        if (vmode == 0) return s[k];
+        if (vmode == 1) return (T)k;
+        return (T)(k - vseg.offs[vseg.seg_of(k)]);
     }
 };
 
@@ -60,7 +74,9 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
                                                             T *__restrict__ out_val, int64_t n, int plog, int clog,
                                                             int64_t tiles_per_region,
                                                             const unsigned *__restrict__ prev_cnt,
-                                                            unsigned *__restrict__ cnt, unsigned *__restrict__ err) {
+                                                            unsigned *__restrict__ cnt,
+                                                            const unsigned *__restrict__ tile_ofs,
+                                                            unsigned *__restrict__ err) {
     __shared__ unsigned s_hist[kMaxFan];
     __shared__ unsigned s_start[kMaxFan];
     __shared__ unsigned s_gofs[kMaxFan];
@@ -92,7 +108,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
             int64_t qq;
             if constexpr (FIRST) {
                 qq = src.at(e);
-                v[j] = in_val[e];
+                v[j] = src.value(in_val, e);
             } else {
                 qq = in_idx[e];
                 v[j] = in_val[e];
@@ -122,7 +138,8 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
             const int d = tid * (kMaxFan / kWave) + i;
             s_start[d] = run;
             run += c[i];
-            s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
+            if (tile_ofs) s_gofs[d] = tile_ofs[(int64_t)d * gridDim.x + blockIdx.x];
+            else s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
         }
     }
     __syncthreads();
@@ -153,6 +170,33 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
     }
 }
 
+// Per-tile child counts of the first partition pass (a single region), as
+// a 256 x tiles matrix: the column scan of it gives every tile its slot in
+// every child without atomics, which matter when thousands of tiles share
+// the first pass's few counters.
+__global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int64_t n, int clog,
+                                                            unsigned *__restrict__ hist) {
+    __shared__ unsigned s_hist[kMaxFan];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
+    __syncthreads();
+    const int64_t lo = (int64_t)blockIdx.x * kPartTile;
+    int64_t qv[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; j++) {
+        const int64_t e = lo + (int64_t)j * kPartThreads + tid;
+        qv[j] = e < n ? src.at(e) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kPartItems; j++) {
+        const int64_t qq = qv[j];
+        const int64_t d = qq >> clog;
+        if (qq >= 0 && qq < n && d < kMaxFan) atomicAdd(&s_hist[d], 1u);
+    }
+    __syncthreads();
+    for (int d = tid; d < kMaxFan; d += kPartThreads) hist[(int64_t)d * gridDim.x + blockIdx.x] = s_hist[d];
+}
+
 // Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
 // first cnt[c] entries of the same range of (idx, val).
 template <class T, bool DIRECT>
@@ -174,7 +218,7 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
         T v;
         if constexpr (DIRECT) {
             q = src.at(base + i);
-            v = in_val[base + i];
+            v = src.value(in_val, base + i);
         } else {
             q = in_idx[base + i];
             v = in_val[base + i];
@@ -198,14 +242,13 @@ static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 // partition pass (at most 3 passes for n < 2^31).
 int64_t partition_permute_scratch_bytes(int64_t n, int width) {
     const int64_t pair = up256(4 * n) + up256((int64_t)width * n);
-    const int64_t counters = 3 * up256(4 * ((n >> kBlkLog) + 2));
-    return 2 * pair + counters + 256;
+    const int64_t counters = 3 * up256(4 * ((n >> kBlkLog) + 2 + kMaxFan));
+    const int64_t hist = up256(4 * (int64_t)kMaxFan * ((n + kPartTile - 1) / kPartTile + 1));
+    return 2 * pair + counters + hist + 256;
 }
 
 template <class T>
-void partition_permute(void *d, const void *s, const int *p, const void *sd, int64_t n, int64_t m, void *scratch) {
-    if (n <= 0) return;
-    IndexSource src{p, SegdView(sd, n, sd ? m : 0), sd != nullptr && m > 1};
+static void run_partition(void *d, const void *s, IndexSource src, int64_t n, void *scratch) {
     unsigned *err = device_error_word();
     // Region sizes per pass: from ceil-pow2(n) down to kBlk, at most 8 bits
     // per pass.
@@ -234,20 +277,29 @@ void partition_permute(void *d, const void *s, const int *p, const void *sd, int
         const int clog = plog - bits;
         const int64_t nchild = ((n - 1) >> clog) + 1;
         prev = cnt;
-        cnt = (unsigned *)b;
-        b += up(4 * (nchild + 1));
-        fill_u32(cnt, 0, nchild + 1);
+        cnt = (unsigned *)b;                       // >= 256 entries: the column scan writes 256 totals
+        const int64_t ncnt = (nchild > kMaxFan ? nchild : kMaxFan) + 1;
+        b += up(4 * ncnt);
+        fill_u32(cnt, 0, ncnt);
         const int64_t regions = ((n - 1) >> plog) + 1;
         const int64_t rsize = (int64_t)1 << plog;
         const int64_t tpr = (rsize < n ? rsize + kPartTile - 1 : n + kPartTile - 1) / kPartTile;
         const int64_t grid = regions * tpr;
         const int out = cur == 0 ? 1 : 0;
-        if (cur < 0)
+        if (cur < 0) {
+            // first pass: one region, many tiles -> slots from a histogram + column scan
+            unsigned *hist = (unsigned *)b;
+            b += up(4 * (int64_t)kMaxFan * grid);
+            k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
+            LAUNCHCHECK();
+            k_col_scan<<<kMaxFan, kColScanThreads, 0, g_stream>>>(hist, grid, cnt);
+            LAUNCHCHECK();
             k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, nullptr, (const T *)s, idx[out], val[out], n, plog, clog, tpr, nullptr, cnt, err);
-        else
+                src, nullptr, (const T *)s, idx[out], val[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+        } else {
             k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, idx[cur], val[cur], idx[out], val[out], n, plog, clog, tpr, prev, cnt, err);
+                src, idx[cur], val[cur], idx[out], val[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
+        }
         LAUNCHCHECK();
         cur = out;
         plog = clog;
@@ -256,6 +308,20 @@ void partition_permute(void *d, const void *s, const int *p, const void *sd, int
     k_block_scatter<T, false><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(src, idx[cur], val[cur], (T *)d, n,
                                                                               cnt, err);
     LAUNCHCHECK();
+}
+
+template <class T>
+void partition_permute(void *d, const void *s, const int *p, const void *sd, int64_t n, int64_t m, void *scratch) {
+    if (n <= 0) return;
+    IndexSource src{p, SegdView(sd, n, sd ? m : 0), sd != nullptr && m > 1, 0, SegdView()};
+    run_partition<T>(d, s, src, n, scratch);
+}
+
+void partition_invert(int *rank, const int *order, int64_t n, const void *sd, int64_t m, void *scratch) {
+    if (n <= 0) return;
+    const bool rel = sd != nullptr && m > 1;
+    IndexSource src{order, SegdView(), false, rel ? 2 : 1, rel ? SegdView(sd, n, m) : SegdView()};
+    run_partition<int>(rank, nullptr, src, n, scratch);
 }
 
 template void partition_permute<int>(void *, const void *, const int *, const void *, int64_t, int64_t, void *);

@@ -10,22 +10,35 @@
 // without changing the result: one histogram pass reads all keys first and
 // the host launches only the passes that reorder something.
 //
-// Each reordering pass is one "onesweep" kernel: a tile of 4096 elements
-// ranks its digits stably with a wave-level multisplit (8 ballots give each
-// lane the lanes holding the same digit), gets each digit's count in earlier
-// tiles from a per-digit decoupled look-back, and scatters (key, index)
-// straight to the next buffer; the last pass writes rank[index] instead.
+// Each reordering pass is a histogram kernel (per-tile digit counts), a
+// column scan of those counts (each tile's offset in each digit's bucket,
+// no atomics and no look-back chain) and a scatter kernel: a tile of 8192
+// elements ranks its digits stably with a wave-level multisplit (8 ballots
+// give each lane the lanes holding the same digit), stages the tile in LDS in
+// digit order and writes each digit's run contiguously (keys, then
+// indices).  The last pass leaves the sorted order of indices; the rank is
+// its inverse, rank[order[p]] = p (minus the segment offset), computed by
+// the coalesced permutation engine of partition.hip.
 #include "cvl_hip.hpp"
 #include "segd.hpp"
 #include "lookback.hpp"
+#include "partition.hpp"
+#include "colscan.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
 
 constexpr int kRadix = 256;
-constexpr int kRankItems = 16;                           // per lane
-constexpr int kRankTile = kBlock * kRankItems;           // 4096
-constexpr int kWaveSpan = kWave * kRankItems;            // 1024 elements per wave
+#ifndef CVLHIP_RANK_THREADS
+#define CVLHIP_RANK_THREADS 1024
+#endif
+#ifndef CVLHIP_RANK_ITEMS
+#define CVLHIP_RANK_ITEMS 8
+#endif
+constexpr int kRankThreads = CVLHIP_RANK_THREADS;        // waves of 64
+constexpr int kRankItems = CVLHIP_RANK_ITEMS;            // per lane
+constexpr int kRankTile = kRankThreads * kRankItems;     // keys per tile
+constexpr int kWaveSpan = kWave * kRankItems;            // consecutive keys per wave
 
 // ---------------------------------------------------------------- key transforms
 // KeyT: uint32_t (int ranks) or uint64_t (double ranks, low field in the low
@@ -93,33 +106,70 @@ struct PassArgs {
     const void *src;         // first pass: the original values; else keys_in
     const int *idx_in;       // null on the first pass (index = position)
     void *keys_out;          // null when the next passes need no keys
-    int *idx_out;            // null on the last pass
-    int *rank;               // non-null on the last pass
+    int *idx_out;            // the element order after this pass
     const unsigned *starts;  // 256 bucket starts of this pass
-    uint64_t *status;        // tiles x 256 look-back words
-    unsigned *counter;
+    const unsigned *tile_ofs;  // 256 x tiles: elements of each digit in earlier tiles
     int64_t n;
     int shift;               // digit = (field >> shift) & 255
     int source;              // 0: int values, 1: double values, 2: keys32, 3: keys64, 4: segment number
     int up;
-    uint32_t flag_agg, flag_pre;   // look-back flags of this pass
     SegdView seg;                  // segment of an element (segmented ranks)
-    int subtract_offset;           // last pass of a segmented rank
     unsigned *err;                 // device error word
 };
 
+// Key of element e for this pass (ix = its original index).
 template <class KeyT>
-__global__ __launch_bounds__(kBlock) void k_rank_pass(PassArgs a) {
-    __shared__ unsigned s_tile;
-    __shared__ unsigned s_cnt[kBlock / kWave][kRadix];
-    __shared__ unsigned s_base[kBlock / kWave][kRadix];   // per wave, per digit: global start
+__device__ __forceinline__ KeyT pass_key(const PassArgs &a, int64_t e, int ix) {
+    if (a.source == 0) return (KeyT)int_key(((const int *)a.src)[e], a.up);
+    if (a.source == 1) return (KeyT)dbl_key(((const double *)a.src)[e], a.up);
+    if (a.source == 2) return (KeyT)((const uint32_t *)a.src)[e];
+    if (a.source == 3) return (KeyT)((const uint64_t *)a.src)[e];
+    return (KeyT)(uint32_t)a.seg.seg_of(ix);
+}
 
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.counter, 1u);
-    for (int i = threadIdx.x; i < (kBlock / kWave) * kRadix; i += kBlock) (&s_cnt[0][0])[i] = 0;
+// Per-tile digit counts of one pass, as a 256 x tiles matrix (tile t =
+// elements [t*8192, (t+1)*8192) of the current order).
+template <class KeyT>
+__global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned *__restrict__ hist) {
+    __shared__ unsigned s_hist[kRadix];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kRadix; i += kRankThreads) s_hist[i] = 0;
     __syncthreads();
-    const int64_t tile = s_tile;
-    const int64_t wb = tile * kRankTile + (int64_t)w * kWaveSpan;
+    const int64_t t0 = (int64_t)blockIdx.x * kRankTile;
+    KeyT kv[kRankItems];
+#pragma unroll
+    for (int j = 0; j < kRankItems; j++) {
+        const int64_t e = t0 + (int64_t)j * kRankThreads + tid;
+        kv[j] = 0;
+        if (e < a.n) kv[j] = pass_key<KeyT>(a, e, a.source == 4 ? (a.idx_in ? a.idx_in[e] : (int)e) : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kRankItems; j++) {
+        const int64_t e = t0 + (int64_t)j * kRankThreads + tid;
+        if (e < a.n) atomicAdd(&s_hist[(kv[j] >> a.shift) & 255], 1u);
+    }
+    __syncthreads();
+    for (int d = tid; d < kRadix; d += kRankThreads) hist[(int64_t)d * gridDim.x + blockIdx.x] = s_hist[d];
+}
+
+template <class KeyT>
+__global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
+    constexpr int W = kRankThreads / kWave;
+    __shared__ unsigned s_cnt[W][kRadix];     // per-wave digit counts, then per-wave starts within a digit
+    __shared__ unsigned s_tot[kRadix];
+    __shared__ unsigned s_lstart[kRadix];     // tile-local start of each digit's run
+    __shared__ unsigned s_gbase[kRadix];      // global position of the run
+    __shared__ KeyT s_stage[kRankTile];       // keys, then (reinterpreted) indices, in digit order
+    __shared__ unsigned char s_dig[kRankTile];
+
+    const int tid = threadIdx.x;
+    const int w = tid / kWave, lane = tid % kWave;
+    for (int i = tid; i < W * kRadix; i += kRankThreads) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t tile = blockIdx.x;
+    const int64_t t0 = tile * kRankTile;
+    const int64_t count = a.n - t0 < kRankTile ? a.n - t0 : kRankTile;
+    const int64_t wb = t0 + (int64_t)w * kWaveSpan;
 
     KeyT key[kRankItems];
     int idx[kRankItems];
@@ -135,11 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_pass(PassArgs a) {
         int ix = 0;
         if (in) {
             ix = a.idx_in ? a.idx_in[e] : (int)e;
-            if (a.source == 0) k = (KeyT)int_key(((const int *)a.src)[e], a.up);
-            else if (a.source == 1) k = (KeyT)dbl_key(((const double *)a.src)[e], a.up);
-            else if (a.source == 2) k = (KeyT)((const uint32_t *)a.src)[e];
-            else if (a.source == 3) k = (KeyT)((const uint64_t *)a.src)[e];
-            else k = (KeyT)(uint32_t)a.seg.seg_of(ix);
+            k = pass_key<KeyT>(a, e, ix);
         }
         key[j] = k;
         idx[j] = ix;
@@ -161,55 +207,71 @@ __global__ __launch_bounds__(kBlock) void k_rank_pass(PassArgs a) {
     }
     __syncthreads();
 
-    // Per digit: counts of the 4 waves, the tile count, and the look-back.
-    {
-        const int d = threadIdx.x;     // kBlock == kRadix
-        unsigned c[kBlock / kWave], tot = 0;
+    // Per digit: wave starts within the digit, the tile count, and the
+    // global start of the tile's run (bucket start + earlier tiles).
+    if (tid < kRadix) {
+        const int d = tid;
+        unsigned run = 0;
 #pragma unroll
-        for (int i = 0; i < kBlock / kWave; i++) { c[i] = tot; tot += s_cnt[i][d]; }
-        uint64_t *st = a.status + tile * kRadix + d;
-        unsigned prefix = 0;
-        if (tile == 0) {
-            st_status(st, ((uint64_t)a.flag_pre << 32) | tot);
-        } else {
-            st_status(st, ((uint64_t)a.flag_agg << 32) | tot);
-            for (int64_t j = tile - 1; j >= 0; j--) {
-                const uint64_t *q = a.status + j * kRadix + d;
-                uint64_t v = ld_status(q);
-                for (int spins = 0; (uint32_t)(v >> 32) < a.flag_agg; spins++) {
-                    if (spins >= kSpinLimit) {           // corrupted status: give up, flag it
-                        atomicOr(a.err, 2u);
-                        v = ((uint64_t)a.flag_pre << 32);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    v = ld_status(q);
-                }
-                prefix += (uint32_t)v;
-                if ((uint32_t)(v >> 32) == a.flag_pre) break;
-            }
-            st_status(st, ((uint64_t)a.flag_pre << 32) | (prefix + tot));
+        for (int i = 0; i < W; i++) {
+            const unsigned c = s_cnt[i][d];
+            s_cnt[i][d] = run;
+            run += c;
         }
-        const unsigned g0 = a.starts[d] + prefix;
+        s_tot[d] = run;
+        const unsigned prefix = a.tile_ofs[(int64_t)d * gridDim.x + tile];
+        s_gbase[d] = a.starts[d] + prefix;
+    }
+    __syncthreads();
+    // tile-local digit starts: exclusive scan of the 256 totals (one wave)
+    if (tid < kWave) {
+        unsigned c[kRadix / kWave], t = 0;
 #pragma unroll
-        for (int i = 0; i < kBlock / kWave; i++) s_base[i][d] = g0 + c[i];
+        for (int i = 0; i < kRadix / kWave; i++) { c[i] = s_tot[tid * (kRadix / kWave) + i]; t += c[i]; }
+        unsigned run = wave_inclusive_scan(t, AddU()) - t;
+#pragma unroll
+        for (int i = 0; i < kRadix / kWave; i++) {
+            s_lstart[tid * (kRadix / kWave) + i] = run;
+            run += c[i];
+        }
     }
     __syncthreads();
 
+    // Stage in digit order, then write each digit's run contiguously:
+    // first the keys (when a later pass needs them), then the indices.
+    unsigned slot[kRankItems];
 #pragma unroll
     for (int j = 0; j < kRankItems; j++) {
         const int64_t e = wb + (int64_t)j * kWave + lane;
-        if (e >= a.n) continue;
-        const int64_t pos = (int64_t)s_base[w][dig[j]] + loc[j];
-        if (pos < 0 || pos >= a.n) { atomicOr(a.err, 1u); continue; }
-        if (a.rank) {
-            int64_t r = pos;
-            if (a.subtract_offset) r -= a.seg.offs[a.seg.seg_of(idx[j])];
-            a.rank[idx[j]] = (int)r;
-        } else {
-            if (a.keys_out) ((KeyT *)a.keys_out)[pos] = key[j];
-            a.idx_out[pos] = idx[j];
+        slot[j] = 0;
+        if (e < a.n) {
+            slot[j] = s_lstart[dig[j]] + s_cnt[w][dig[j]] + loc[j];
+            s_dig[slot[j]] = (unsigned char)dig[j];
+            if (a.keys_out) s_stage[slot[j]] = key[j];
         }
+    }
+    __syncthreads();
+    if (a.keys_out) {
+        for (int64_t i = tid; i < count; i += kRankThreads) {
+            const unsigned d = s_dig[i];
+            const int64_t pos = (int64_t)s_gbase[d] + (i - s_lstart[d]);
+            if (pos < a.n) ((KeyT *)a.keys_out)[pos] = s_stage[i];
+            else atomicOr(a.err, 1u);
+        }
+        __syncthreads();
+    }
+    int *s_idx = (int *)s_stage;
+#pragma unroll
+    for (int j = 0; j < kRankItems; j++) {
+        const int64_t e = wb + (int64_t)j * kWave + lane;
+        if (e < a.n) s_idx[slot[j]] = idx[j];
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < count; i += kRankThreads) {
+        const unsigned d = s_dig[i];
+        const int64_t pos = (int64_t)s_gbase[d] + (i - s_lstart[d]);
+        if (pos < a.n) a.idx_out[pos] = s_idx[i];
+        else atomicOr(a.err, 1u);
     }
 }
 
@@ -221,7 +283,7 @@ __global__ void k_rank_identity(int *rank, int64_t n, SegdView seg, int segmente
 
 // ---------------------------------------------------------------- host driver
 struct RankScratch {
-    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, bytes;
+    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, invert, bytes;
     RankScratch(int64_t n, bool dbl) {
         auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
         int64_t kw = dbl ? 8 : 4;
@@ -234,7 +296,8 @@ struct RankScratch {
         starts = up(hist + 4 * 12 * kRadix);
         counters = up(starts + 4 * 12 * kRadix);
         status = up(counters + 4 * 16);
-        bytes = up(status + 8 * kRadix * (tiles > 0 ? tiles : 1));
+        invert = up(status + 4 * kRadix * (tiles > 0 ? tiles : 1));
+        bytes = up(invert + partition_permute_scratch_bytes(n, 4));
     }
 };
 
@@ -248,8 +311,7 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     char *b = (char *)scratch;
     unsigned *hist = (unsigned *)(b + L.hist);
     unsigned *starts = (unsigned *)(b + L.starts);
-    unsigned *counters = (unsigned *)(b + L.counters);
-    uint64_t *status = (uint64_t *)(b + L.status);
+    unsigned *tile_ofs = (unsigned *)(b + L.status);
     const bool segmented = sd != nullptr && m > 1;
     SegdView seg(sd, n, segmented ? m : 0);
     const int key_passes = dbl ? 8 : 4;
@@ -262,8 +324,6 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     const int64_t tiles = (n + kRankTile - 1) / kRankTile;
 
     fill_u32(hist, 0, 12 * kRadix);
-    fill_u32(counters, 0, 16);
-    fill_u32(status, 0, 2 * kRadix * tiles);
     {
         int g = grid_for(n, kBlock * 16);
         if (g > 2048) g = 2048;
@@ -304,17 +364,13 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     int cur = -1;   // buffer holding the current (keys, idx); -1 = original input
     for (int i = 0; i < nrun; i++) {
         const int p = run[i];
-        const bool last = i == nrun - 1;
         const int out = cur == 0 ? 1 : 0;
         PassArgs a;
         a.n = n;
         a.up = up;
         a.seg = seg;
         a.starts = starts + p * kRadix;
-        a.status = status;
-        a.counter = counters + i;
-        a.flag_agg = 2 * i + 1;
-        a.flag_pre = 2 * i + 2;
+        a.tile_ofs = tile_ofs;
         a.idx_in = cur < 0 ? nullptr : idxs[cur];
         if (p < key_passes) {
             a.source = cur < 0 ? (dbl ? 1 : 0) : (dbl ? 3 : 2);
@@ -325,16 +381,21 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
             a.src = nullptr;
             a.shift = 8 * (p - key_passes);
         }
-        a.keys_out = (!last && i < last_key_run) ? keys[out] : nullptr;
-        a.idx_out = last ? nullptr : idxs[out];
-        a.rank = last ? rank : nullptr;
-        a.subtract_offset = last && segmented;
+        a.keys_out = i < last_key_run ? keys[out] : nullptr;
+        a.idx_out = idxs[out];
         a.err = device_error_word();
-        if (dbl) k_rank_pass<uint64_t><<<(unsigned)tiles, kBlock, 0, g_stream>>>(a);
-        else k_rank_pass<uint32_t><<<(unsigned)tiles, kBlock, 0, g_stream>>>(a);
+        if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
+        else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
+        LAUNCHCHECK();
+        k_col_scan<<<kRadix, kColScanThreads, 0, g_stream>>>(tile_ofs, tiles, nullptr);
+        LAUNCHCHECK();
+        if (dbl) k_rank_pass<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
+        else k_rank_pass<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
         LAUNCHCHECK();
         cur = out;
     }
+    // rank = inverse of the final order
+    partition_invert(rank, idxs[cur], n, segmented ? sd : nullptr, m, b + L.invert);
 }
 
 int64_t rank_scratch_bytes(int64_t n, bool dbl) { return RankScratch(n, dbl).bytes; }
