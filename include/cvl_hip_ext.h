@@ -46,6 +46,34 @@ unsigned long long cvlhip_rand_get_state(void);
 void cvlhip_fill_splitmix_u32(vec_p d, long long n, unsigned long long seed);
 void cvlhip_fill_splitmix_f64(vec_p d, long long n, unsigned long long seed, double lo, double hi);
 
+/* ---- segmented-op schedules ---------------------------------------- */
+/* A segmented scan / reduction can run under several schedules ("arms"):
+ * 0 flat 16-row tiles, 1 flat 8-row tiles, 2 thread per segment, 3 wave per
+ * segment, 4 block (256) per segment, 5 block (1024) per segment.  op: 0 +,
+ * 1 *, 2 min, 3 max; ty: 0 int, 1 double.  Same arguments and results as the
+ * CVL functions (e.g. add_sed / max_red); scratch: cvlhip_seg_scratch units. */
+int cvlhip_seg_arms(void);
+void cvlhip_seg_scan(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
+void cvlhip_seg_reduce(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
+int cvlhip_seg_scratch(int n, int m);
+/* Descriptor statistics for the dispatcher (host floats, see segstrat.hip). */
+int cvlhip_seg_features_count(void);
+void cvlhip_segd_features(vec_p sd, int n, int m, float *out);
+
+/* ---- learned dispatcher network (device pointers, fp32) -------------- */
+/* Sizes: F input features, H hidden units (two ReLU layers), A outputs (one
+ * predicted log-time per schedule), P parameters in the flat array. */
+void cvlhip_mlp_dims(int *features, int *hidden, int *arms, int *params);
+/* out: B*A predictions; acts: 2*B*H floats kept for the backward pass. */
+void cvlhip_mlp_forward(const float *params, const float *X, int B, float *out, float *acts);
+/* Masked mean-squared-error gradients of one batch: T targets, M mask
+ * (B*A each), mask_sum = sum of M; grads: P floats; loss: 1 float;
+ * acts: 2*B*H floats; work: B*(2H+2A) floats. */
+void cvlhip_mlp_gradients(const float *params, const float *X, const float *T, const float *M, int B,
+                          float mask_sum, float *grads, float *loss, float *acts, float *work);
+/* One Adam step, t >= 1. */
+void cvlhip_mlp_adam(float *params, const float *grads, float *m, float *v, int t, float lr);
+
 #ifdef __cplusplus
 }
 #endif

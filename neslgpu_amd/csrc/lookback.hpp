@@ -69,6 +69,15 @@ constexpr int kLookK = CVLHIP_LOOKBACK_K;
 // Values are combined in tile order (earliest operand first), so
 // order-sensitive operators (double max/min with signed zeros) agree with a
 // left fold.  Tiles before tile 0 read as a prefix equal to the neutral.
+// Operators whose result depends on association (double + and *).  Their
+// look-back folds the window strictly in tile order and never slides the
+// window (it re-polls until a prefix appears in it), so every inclusive
+// prefix equals the sequential fold of the tile aggregates: the result does
+// not depend on timing and is the same on every run.
+template <class Op> struct seq_combine { static constexpr bool value = false; };
+template <> struct seq_combine<AddD> { static constexpr bool value = true; };
+template <> struct seq_combine<MulD> { static constexpr bool value = true; };
+
 template <class Op>
 __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile, Op op,
                                    unsigned *__restrict__ err) {
@@ -78,7 +87,7 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
     const int lane = lane_id();
     T acc = Op::neutral();
     int64_t pos = tile - 1;
-    for (;;) {
+    for (int rounds = 0;; rounds++) {
         T v[K];
         uint32_t f[K];
 #pragma unroll
@@ -118,6 +127,17 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
 #pragma unroll
         for (int j = K - 1; j >= 0; j--)
             if (j <= lim) c = op(c, v[j]);
+        if constexpr (seq_combine<Op>::value) {
+            if (!pm && rounds < kSpinLimit) {      // re-poll the same window
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            if (!pm && lane == 0) atomicOr(err, 2u);
+            const int Ls = L < kWave ? L : kWave - 1;
+            T w = shfl(c, Ls);                     // the prefix, then later tiles in order
+            for (int l = Ls - 1; l >= 0; l--) w = op(w, shfl(c, l));
+            return w;
+        }
         T r = shfl(c, kWave - 1 - lane);          // lane 0 now holds the earliest tiles
         acc = op(wave_reduce(r, op), acc);
         if (pm) break;

@@ -352,9 +352,9 @@ int64_t scan_scratch_bytes(int64_t n) {
 
 static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
-template <class Op, bool SEG, bool REDUCE>
-static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
-    using S = ScanShape<Op, kScanRows>;
+template <class Op, int R, bool SEG, bool REDUCE>
+static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+    using S = ScanShape<Op, R>;
     if (n <= 0) return;
     if (!scratch) usage_error("scan of %lld elements called without scratch", (long long)n);
     int64_t tiles = (n + S::TILE - 1) / S::TILE;
@@ -363,9 +363,14 @@ static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scr
     unsigned *counter = (unsigned *)scratch;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
     int aligned = al16(s) && al16(d);
-    k_scan<Op, kScanRows, SEG, REDUCE><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
+    k_scan<Op, R, SEG, REDUCE><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
         (const typename Op::T *)s, (typename Op::T *)d, n, sb, status, counter, aligned, device_error_word());
     LAUNCHCHECK();
+}
+
+template <class Op, bool SEG, bool REDUCE>
+static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+    launch_scan_r<Op, kScanRows, SEG, REDUCE>(s, d, n, sb, scratch);
 }
 
 template <class Op>
@@ -439,12 +444,48 @@ void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, vo
     launch_scan<Op, true, true>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
 }
 
+// The same with 8-row tiles (a schedule the segmented-op dispatcher may
+// pick; the scratch it needs is twice that of the default tiles).
+template <class Op>
+void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+    if (m == 1) { launch_scan_r<Op, 8, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
+    SegdView v(sd, n, m);
+    launch_scan_r<Op, 8, true, false>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+}
+
+template <class Op>
+void reduce_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+    using T = typename Op::T;
+    if (m <= 0) return;
+    if (m == 1) { reduce_unseg_dev<Op>(d, s, n, scratch); return; }
+    SegdView v(sd, n, m);
+    int g = grid_for(m, kBlock);
+    if (g > 4096) g = 4096;
+    k_empty_identity<T><<<g, kBlock, 0, g_stream>>>((T *)d, v.lens, m, Op::cvl_id());
+    LAUNCHCHECK();
+    launch_scan_r<Op, 8, true, true>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+}
+
+#define CVLHIP_INST_SEG(OP)                                                                         \
+    template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);      \
+    template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);    \
+    template void scan_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);   \
+    template void reduce_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);
+CVLHIP_INST_SEG(AddZ)
+CVLHIP_INST_SEG(MulZ)
+CVLHIP_INST_SEG(MinZ)
+CVLHIP_INST_SEG(MaxZ)
+CVLHIP_INST_SEG(AddD)
+CVLHIP_INST_SEG(MulD)
+CVLHIP_INST_SEG(MinD)
+CVLHIP_INST_SEG(MaxD)
+template int64_t scan_scratch_bytes<AddD>(int64_t);
+
 // Explicit instantiations used by other translation units.
 template void scan_unseg<AddZ>(void *, const void *, int64_t, void *);
 template void scan_unseg<CountB>(void *, const void *, int64_t, void *);
 template void reduce_seg<CountB>(void *, const void *, const void *, int64_t, int64_t, void *);
 template CountB::T reduce_unseg<CountB>(const void *, int64_t, void *);
-template void reduce_seg<AddZ>(void *, const void *, const void *, int64_t, int64_t, void *);
 template int64_t scan_scratch_bytes<AddZ>(int64_t);
 template AddZ::T reduce_unseg<AddZ>(const void *, int64_t, void *);
 

@@ -32,6 +32,7 @@ _CTYPES = {
     "void": None, "int": C.c_int, "unsigned int": C.c_uint, "unsigned": C.c_uint,
     "double": C.c_double, "cvl_bool": C.c_int, "vec_p": C.c_void_p, "long long": C.c_longlong,
     "unsigned long long": C.c_ulonglong, "char *": C.c_char_p, "void *": C.c_void_p,
+    "float": C.c_float,
 }
 _PTR_ARG = C.c_void_p   # int *, double *, cvl_bool *, cvl_timer_t *: raw addresses
 
@@ -109,7 +110,7 @@ class Lib:
             for a, t in zip(args, ats):
                 if t in (C.c_void_p,):
                     conv.append(_addr(a))
-                elif t is C.c_double:
+                elif t is C.c_double or t is C.c_float:
                     conv.append(float(a))
                 elif t is C.c_char_p:
                     conv.append(a)

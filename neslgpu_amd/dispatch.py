@@ -1,0 +1,263 @@
+"""Scheduler for segmented scans and reductions: exhaustive autotuner and a
+learned dispatcher.
+
+A segmented operation can run under several schedules ("arms", see
+csrc/segstrat.hip): flat look-back tiles of two sizes, thread-, wave- or
+block-per-segment.  The autotuner times every arm on a workload (HIP
+events); the dispatcher is a small MLP that reads device-computed
+descriptor statistics (n, m, log2-length histograms by elements and by
+segments, dtype, scan/reduce) and predicts every arm's log-time; it picks
+the arm with the lowest prediction.  The network's forward and backward
+passes and its Adam steps are HIP kernels of libcvl_hip.so
+(cvlhip_mlp_*); this module only orchestrates them.  Only the schedule
+changes with the choice, never the semantics; the autotuner stays the
+fallback and the judge of the dispatcher's regret.
+
+    python -m neslgpu_amd.dispatch train   # gather timings, train, save, evaluate
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(HERE, "dispatch_model.npz")
+
+ARMS = ["flat16", "flat8", "thread", "wave", "block256", "block1024"]
+OPS = {"add": 0, "mul": 1, "min": 2, "max": 3}
+
+
+def mlp_dims(lib):
+    a = [C.c_int() for _ in range(4)]
+    lib.dll.cvlhip_mlp_dims(*[C.byref(x) for x in a])
+    return tuple(x.value for x in a)
+
+
+# ------------------------------------------------------------------ workloads
+def lengths(kind: str, n: int, param: float, rng: np.random.Generator, lmax: int = 1 << 20) -> np.ndarray:
+    """Segment lengths summing exactly to n."""
+    out, tot = [], 0
+    while tot < n:
+        k = max(1024, int(2 * (n - tot) / max(param, 1.0)) + 16)
+        if kind == "geometric":
+            z = rng.geometric(1.0 / param, size=k)
+        elif kind == "uniform":
+            z = rng.integers(1, int(2 * param) + 1, size=k)
+        elif kind == "zipf":
+            z = rng.zipf(param, size=k)
+        elif kind == "const":
+            z = np.full(k, int(param))
+        elif kind == "bimodal":            # many short segments and a few long ones
+            z = np.where(rng.random(k) < 0.002, rng.integers(1 << 14, 1 << 18, size=k), rng.integers(1, 8, size=k))
+        else:
+            raise ValueError(kind)
+        z = np.minimum(z, lmax)
+        out.append(z)
+        tot += int(z.sum())
+    L = np.concatenate(out).astype(np.int64)
+    cs = np.cumsum(L)
+    k = int(np.searchsorted(cs, n))
+    L = L[: k + 1].copy()
+    L[-1] -= int(cs[k] - n)
+    return L[L > 0].astype(np.int32)
+
+
+TRAIN_WORKLOADS = (
+    [("geometric", m) for m in (2, 4, 16, 64, 256, 1024, 8192)]
+    + [("uniform", m) for m in (3, 24, 200, 3000)]
+    + [("zipf", a) for a in (1.3, 1.7, 2.2, 3.0)]
+    + [("const", c) for c in (1, 32, 100000)]
+    + [("bimodal", 0)]
+)
+EVAL_WORKLOADS = [("zipf", a) for a in (1.2, 1.5, 1.8, 2.5)]
+
+
+# ------------------------------------------------------------------ timing
+class Autotuner:
+    """Times every arm of a segmented op (median of `reps` runs, HIP events)."""
+
+    def __init__(self, cv, reps: int = 3, slow_ms: float = 50.0):
+        import torch
+        self.cv, self.torch, self.reps, self.slow_ms = cv, torch, reps, slow_ms
+
+    def run(self, kind, op, ty, arm, d, s, sd):
+        L = self.cv.lib
+        scr = self.cv.scratch(L.cvlhip_seg_scratch(sd.n, sd.m))
+        f = L.cvlhip_seg_scan if kind == "scan" else L.cvlhip_seg_reduce
+        f(OPS[op], 1 if ty == "d" else 0, arm, d, s, sd, sd.n, sd.m, scr)
+
+    def time_arm(self, kind, op, ty, arm, d, s, sd) -> float:
+        torch = self.torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        times = []
+        for r in range(self.reps + 1):
+            torch.cuda.synchronize()
+            e0.record()
+            self.run(kind, op, ty, arm, d, s, sd)
+            e1.record()
+            e1.synchronize()
+            t = e0.elapsed_time(e1)
+            if r > 0:
+                times.append(t)
+            if t > self.slow_ms:          # a hopeless arm: one timing is enough
+                return t
+        return float(np.median(times))
+
+    def tune(self, kind, op, ty, d, s, sd) -> np.ndarray:
+        return np.array([self.time_arm(kind, op, ty, a, d, s, sd) for a in range(len(ARMS))])
+
+
+def features(cv, sd, kind: str, ty: str) -> np.ndarray:
+    F = cv.lib.cvlhip_seg_features_count()
+    out = np.zeros(F, np.float32)
+    cv.lib.cvlhip_segd_features(sd, sd.n, sd.m, out)
+    out[F - 2] = 1.0 if ty == "d" else 0.0
+    out[F - 1] = 1.0 if kind == "reduce" else 0.0
+    return out
+
+
+# ------------------------------------------------------------------ the network
+class Dispatcher:
+    """MLP over descriptor statistics -> predicted log-time per arm."""
+
+    def __init__(self, cv, seed: int = 0):
+        import torch
+        self.cv, self.torch = cv, torch
+        self.F, self.H, self.A, self.P = mlp_dims(cv.lib)
+        g = torch.Generator().manual_seed(seed)
+        p = torch.zeros(self.P, dtype=torch.float32)
+        o = 0
+        for fan_in, fan_out in ((self.F, self.H), (self.H, self.H), (self.H, self.A)):
+            w = torch.randn(fan_out * fan_in, generator=g) * math.sqrt(2.0 / fan_in)
+            p[o:o + w.numel()] = w
+            o += w.numel() + fan_out          # biases start at 0
+        self.params = p.to(cv.device)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+
+    def _dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(self.cv.device)
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        torch = self.torch
+        B = X.shape[0]
+        Xd = self._dev(X)
+        out = torch.empty(B * self.A, dtype=torch.float32, device=self.cv.device)
+        acts = torch.empty(2 * B * self.H, dtype=torch.float32, device=self.cv.device)
+        self.cv.lib.cvlhip_mlp_forward(self.params, Xd, B, out, acts)
+        return out.cpu().numpy().reshape(B, self.A)
+
+    def gradients(self, X, T, M):
+        """(loss, grads) of the masked MSE on one batch, from the HIP kernels."""
+        torch = self.torch
+        B = X.shape[0]
+        grads = torch.zeros(self.P, dtype=torch.float32, device=self.cv.device)
+        loss = torch.zeros(1, dtype=torch.float32, device=self.cv.device)
+        acts = torch.empty(2 * B * self.H, dtype=torch.float32, device=self.cv.device)
+        work = torch.empty(B * (2 * self.H + 2 * self.A), dtype=torch.float32, device=self.cv.device)
+        self.cv.lib.cvlhip_mlp_gradients(self.params, self._dev(X), self._dev(T), self._dev(M), B,
+                                         float(np.sum(M)), grads, loss, acts, work)
+        return float(loss.item()), grads
+
+    def train(self, X, T, M, epochs: int = 2000, lr: float = 3e-3, batch: int = 64, seed: int = 0):
+        rng = np.random.default_rng(seed)
+        N = X.shape[0]
+        losses = []
+        for ep in range(epochs):
+            idx = rng.permutation(N)[: min(batch, N)]
+            loss, g = self.gradients(X[idx], T[idx], M[idx])
+            self.step += 1
+            self.cv.lib.cvlhip_mlp_adam(self.params, g, self.m, self.v, self.step, lr)
+            losses.append(loss)
+        return losses
+
+    def choose(self, feat: np.ndarray) -> int:
+        return int(np.argmin(self.predict(feat[None, :])[0]))
+
+    def save(self, path: str = MODEL_PATH):
+        np.savez(path, params=self.params.cpu().numpy())
+
+    def load(self, path: str = MODEL_PATH) -> "Dispatcher":
+        with np.load(path, allow_pickle=False) as z:
+            self.params = self._dev(z["params"])
+        return self
+
+
+# ------------------------------------------------------------------ data and evaluation
+def gather(cv, workloads, n: int, ops=(("scan", "add", "d"), ("reduce", "max", "d")), seed: int = 0,
+           log=print):
+    """(X, T, M, meta): features and log-times of every arm for every workload."""
+    import torch
+    tuner = Autotuner(cv)
+    rng = np.random.default_rng(seed)
+    x = torch.empty(n, dtype=torch.float64, device=cv.device)
+    cv.lib.cvlhip_fill_splitmix_f64(x, n, 0x4E45534C00000003, -1.0, 1.0)
+    d = torch.empty_like(x)
+    X, T, M, meta = [], [], [], []
+    for (kind, param) in workloads:
+        lens = lengths(kind, n, param, rng)
+        sd = cv.segd(lens)
+        for (k, op, ty) in ops:
+            t = tuner.tune(k, op, ty, d, x, sd)
+            X.append(features(cv, sd, k, ty))
+            T.append(np.log(t))
+            M.append(np.ones(len(ARMS), np.float32))
+            meta.append({"dist": kind, "param": param, "n": n, "m": int(lens.size), "op": f"{op}_{k}",
+                         "ms": [round(float(v), 4) for v in t]})
+            log(json.dumps(meta[-1]))
+    return (np.array(X, np.float32), np.array(T, np.float32), np.array(M, np.float32), meta)
+
+
+def evaluate(disp: Dispatcher, X, T, meta):
+    """Regret of the dispatcher against the autotuner, and totals per arm."""
+    pred = disp.predict(X)
+    times = np.exp(T)
+    pick = pred.argmin(axis=1)
+    best = times.min(axis=1)
+    chosen = times[np.arange(len(X)), pick]
+    per_arm_total = times.sum(axis=0)
+    return {
+        "cases": len(X),
+        "learned_total_ms": float(chosen.sum()),
+        "autotuned_total_ms": float(best.sum()),
+        "regret_total": float(chosen.sum() / best.sum() - 1.0),
+        "max_case_regret": float((chosen / best - 1.0).max()),
+        "fixed_arm_total_ms": {ARMS[a]: float(per_arm_total[a]) for a in range(len(ARMS))},
+        "best_fixed_arm": ARMS[int(per_arm_total.argmin())],
+        "learned_beats_every_fixed_arm": bool(chosen.sum() < per_arm_total.min()),
+        "picks": [ARMS[a] for a in pick],
+    }
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    from .cvl import Cvl
+    cv = Cvl()
+    n_train = 1 << 24
+    n_eval = 1 << 26
+    t0 = time.time()
+    X, T, M, meta = gather(cv, TRAIN_WORKLOADS, n_train)
+    X2, T2, M2, meta2 = gather(cv, TRAIN_WORKLOADS[::3], n_eval, seed=5)
+    Xtr = np.concatenate([X, X2]); Ttr = np.concatenate([T, T2]); Mtr = np.concatenate([M, M2])
+    disp = Dispatcher(cv)
+    losses = disp.train(Xtr, Ttr, Mtr, epochs=3000)
+    disp.save()
+    Xe, Te, Me, metae = gather(cv, EVAL_WORKLOADS, n_eval, seed=11)
+    res = {"train_cases": int(len(Xtr)), "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),
+           "eval": evaluate(disp, Xe, Te, metae), "eval_cases": metae, "seconds": round(time.time() - t0, 1)}
+    print(json.dumps(res), flush=True)
+    out = argv[1] if len(argv) > 1 else None
+    if out:
+        with open(out, "w") as f:
+            json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -37,6 +37,12 @@ def fp_add_tol(absprefix, seglen):
     return 2.0 * (math.ceil(math.log2(L)) + 2) * 2.0 ** -53 * absprefix
 
 
+def fp_add_tol_vec(absprefix, seglen):
+    """Vector form of fp_add_tol."""
+    L = np.maximum(np.asarray(seglen, np.int64), 2)
+    return 2.0 * (np.ceil(np.log2(L)) + 2) * 2.0 ** -53 * np.asarray(absprefix)
+
+
 def seg_lengths_per_element(lens):
     lens = np.asarray(lens, np.int64)
     return np.repeat(lens, lens)
