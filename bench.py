@@ -113,6 +113,7 @@ class Sweep:
             "pk2_luz": 8 * n + 4 * self.kept,
             "smp_puz": 12 * n,
             "rku_luz": 8 * nr,
+            "cpy_wuz": 8 * n,
         }
 
     def run(self, name: str):
@@ -129,6 +130,8 @@ class Sweep:
             L.smp_puz(self.d, self.s, self.perm, n, self.scr)
         elif name == "rku_luz":
             L.rku_luz(self.rank, self.keys, self.nr, self.scr)
+        elif name == "cpy_wuz":            # calibration copy (hipMemcpy device to device)
+            L.cpy_wuz(self.d, self.s, n, self.scr)
 
     def step(self, names):
         for k in names:
@@ -243,7 +246,7 @@ def main():
     sw = Sweep(a.log2n, a.rank_log2n)
     names = ["add_suz", "max_ruz", "add_sez", "pk2_luz", "smp_puz", "rku_luz"]
     if a.only:
-        names = [x for x in names if x in a.only.split(",")]
+        names = [x for x in names + ["cpy_wuz"] if x in a.only.split(",")]
     byts = sw.bytes()
     for _ in range(a.warmup):
         sw.step(names)
@@ -261,7 +264,10 @@ def main():
     dom = max(prims, key=lambda k: prims[k]["ms"])
     kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce (MaxZ)",
                  "add_sez": "k_scan (AddZ, segmented)", "pk2_luz": "k_pack<int>",
-                 "smp_puz": "k_scatter<int>", "rku_luz": "k_rank_pass<uint32_t> (4 passes)"}
+                 "smp_puz": "k_part_hist + k_partition<int> x2 + k_block_scatter<int>",
+                 "rku_luz": "k_key_hist + 4 x (k_rank_hist + k_col_scan + k_rank_pass<uint32_t>) + "
+                            "inversion (k_part_hist + k_partition<int> x2 + k_block_scatter<int>)",
+                 "cpy_wuz": "__amd_rocclr_copyBuffer"}
     roof = {
         "bound": "hbm",
         "kernel": kernel_of[dom],

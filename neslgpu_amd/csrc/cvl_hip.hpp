@@ -246,6 +246,20 @@ __device__ __forceinline__ T wave_reduce(T x, Op op) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// ------------------------------------------------------------------ XCD-aware tiles
+// MI355X spreads consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  xcd_tile maps block b of a grid of g blocks to a tile
+// index such that the blocks of one XCD (b % 8 == x) take a contiguous range
+// of tiles; neighbouring tiles' partial-line writes (histogram columns, the
+// ends of bucket runs) then meet in the same L2 and merge before they reach
+// HBM.  It is a bijection on [0, g).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t g) {
+    const int64_t q = g / kXcds;
+    if (b >= kXcds * q) return b;
+    return (b % kXcds) * q + b / kXcds;
+}
+
 // ------------------------------------------------------------------ coherent status words
 // Tile status words of single-pass scans live in global memory and are
 // written and polled by different workgroups, possibly on different XCDs:

@@ -85,8 +85,9 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
     __shared__ unsigned char s_dig[kPartTile];
 
     const int tid = threadIdx.x;
-    const int64_t region = blockIdx.x / tiles_per_region;
-    const int64_t chunk = blockIdx.x % tiles_per_region;
+    const int64_t bt = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t region = bt / tiles_per_region;
+    const int64_t chunk = bt % tiles_per_region;
     const int64_t rlo = region << plog;
     int64_t rhi = (rlo + ((int64_t)1 << plog)) < n ? rlo + ((int64_t)1 << plog) : n;
     if (!FIRST && rlo + (int64_t)prev_cnt[region] < rhi) rhi = rlo + prev_cnt[region];   // pairs that landed
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
             const int d = tid * (kMaxFan / kWave) + i;
             s_start[d] = run;
             run += c[i];
-            if (tile_ofs) s_gofs[d] = tile_ofs[(int64_t)d * gridDim.x + blockIdx.x];
+            if (tile_ofs) s_gofs[d] = tile_ofs[(int64_t)d * gridDim.x + bt];
             else s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
         }
     }
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
     const int tid = threadIdx.x;
     for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
     __syncthreads();
-    const int64_t lo = (int64_t)blockIdx.x * kPartTile;
+    const int64_t bt = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t lo = bt * kPartTile;
     int64_t qv[kPartItems];
 #pragma unroll
     for (int j = 0; j < kPartItems; j++) {
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
         if (qq >= 0 && qq < n && d < kMaxFan) atomicAdd(&s_hist[d], 1u);
     }
     __syncthreads();
-    for (int d = tid; d < kMaxFan; d += kPartThreads) hist[(int64_t)d * gridDim.x + blockIdx.x] = s_hist[d];
+    for (int d = tid; d < kMaxFan; d += kPartThreads) hist[(int64_t)d * gridDim.x + bt] = s_hist[d];
 }
 
 // Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
@@ -207,12 +209,13 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
     __shared__ T s_out[kBlk];
     __shared__ unsigned s_mask[kBlk / 32];
     const int tid = threadIdx.x;
-    const int64_t base = (int64_t)blockIdx.x * kBlk;
+    const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t base = blk * kBlk;
     const int64_t end = base + kBlk < n ? base + kBlk : n;
     for (int i = tid; i < kBlk / 32; i += kPartThreads) s_mask[i] = 0;
     __syncthreads();
     int64_t count = end - base;
-    if (!DIRECT && (int64_t)cnt[blockIdx.x] < count) count = cnt[blockIdx.x];
+    if (!DIRECT && (int64_t)cnt[blk] < count) count = cnt[blk];
     for (int64_t i = tid; i < count; i += kPartThreads) {
         int64_t q;
         T v;

@@ -135,7 +135,8 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
     const int tid = threadIdx.x;
     for (int i = tid; i < kRadix; i += kRankThreads) s_hist[i] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)blockIdx.x * kRankTile;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t t0 = tile * kRankTile;
     KeyT kv[kRankItems];
 #pragma unroll
     for (int j = 0; j < kRankItems; j++) {
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
         if (e < a.n) atomicAdd(&s_hist[(kv[j] >> a.shift) & 255], 1u);
     }
     __syncthreads();
-    for (int d = tid; d < kRadix; d += kRankThreads) hist[(int64_t)d * gridDim.x + blockIdx.x] = s_hist[d];
+    for (int d = tid; d < kRadix; d += kRankThreads) hist[(int64_t)d * gridDim.x + tile] = s_hist[d];
 }
 
 template <class KeyT>
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     const int w = tid / kWave, lane = tid % kWave;
     for (int i = tid; i < W * kRadix; i += kRankThreads) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = tile * kRankTile;
     const int64_t count = a.n - t0 < kRankTile ? a.n - t0 : kRankTile;
     const int64_t wb = t0 + (int64_t)w * kWaveSpan;
