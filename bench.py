@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--rank-log2n", type=int, default=26)
     p.add_argument("--no-cpu", action="store_true", help="skip the host baseline")
     p.add_argument("--only", default="", help="comma list of primitives to run (profiling)")
+    p.add_argument("--no-irregular", action="store_true",
+                   help="skip the irregular float64 segmented workload and its dispatch comparison")
+    p.add_argument("--irregular-log2n", type=int, default=26)
     return p.parse_args()
 
 
@@ -64,7 +67,9 @@ def geometric_lengths(n: int, mean: int, seed: int) -> np.ndarray:
 
 
 class Sweep:
-    def __init__(self, log2n: int, rank_log2n: int):
+    """Device inputs for the primitives in `names` (only those are built)."""
+
+    def __init__(self, log2n: int, rank_log2n: int, names):
         import torch
         from neslgpu_amd.cvl import Cvl
         self.torch = torch
@@ -73,33 +78,40 @@ class Sweep:
         n = 1 << log2n
         nr = 1 << rank_log2n
         self.n, self.nr = n, nr
+        self.m, self.kept = 0, 0
         dev = self.cv.device
         i32 = torch.int32
         L = self.lib
         # inputs: splitmix64 streams generated on the device
         self.s = torch.empty(n, dtype=i32, device=dev)
         L.cvlhip_fill_splitmix_u32(self.s, n, SEED + CONFIG_ID)
-        self.keys = torch.empty(nr, dtype=i32, device=dev)
-        L.cvlhip_fill_splitmix_u32(self.keys, nr, SEED + CONFIG_ID + 1)
-        u = torch.empty(n, dtype=i32, device=dev)
-        L.cvlhip_fill_splitmix_u32(u, n, SEED + CONFIG_ID + 2)
-        self.flags = (u < 0).to(i32)                       # top bit: Bernoulli(0.5)
-        del u
-        g = torch.Generator(device=dev)
-        g.manual_seed(SEED + CONFIG_ID + 3 & 0x7FFFFFFF)
-        self.perm = torch.randperm(n, device=dev, generator=g, dtype=torch.int64).to(i32)
-        self.lens = geometric_lengths(n, 256, SEED + CONFIG_ID + 4 & 0x7FFFFFFF)
-        self.m = int(self.lens.size)
-        self.segd = self.cv.segd(self.lens)
-        # outputs and scratch
         self.d = torch.empty(n, dtype=i32, device=dev)
-        self.kept = int(self.flags.sum().item())
-        self.rank = torch.empty(nr, dtype=i32, device=dev)
+        units = [1, L.add_suz_scratch(n), L.max_ruz_scratch(n)]
+        if "rku_luz" in names:
+            self.keys = torch.empty(nr, dtype=i32, device=dev)
+            L.cvlhip_fill_splitmix_u32(self.keys, nr, SEED + CONFIG_ID + 1)
+            self.rank = torch.empty(nr, dtype=i32, device=dev)
+            units.append(L.rku_luz_scratch(nr))
+        if "pk2_luz" in names:
+            u = torch.empty(n, dtype=i32, device=dev)
+            L.cvlhip_fill_splitmix_u32(u, n, SEED + CONFIG_ID + 2)
+            self.flags = (u < 0).to(i32)                   # top bit: Bernoulli(0.5)
+            del u
+            self.kept = int(self.flags.sum().item())
+            units.append(L.pk2_luz_scratch(n))
+        if "smp_puz" in names:
+            g = torch.Generator(device=dev)
+            g.manual_seed(SEED + CONFIG_ID + 3 & 0x7FFFFFFF)
+            self.perm = torch.randperm(n, device=dev, generator=g, dtype=torch.int64).to(i32)
+            units.append(L.smp_puz_scratch(n))
+        if "add_sez" in names:
+            self.lens = geometric_lengths(n, 256, SEED + CONFIG_ID + 4 & 0x7FFFFFFF)
+            self.m = int(self.lens.size)
+            self.segd = self.cv.segd(self.lens)
+            units.append(L.add_sez_scratch(n, self.m))
         # one scratch area serves every primitive: the largest of their
         # *_scratch() requirements (CVL units of 8 bytes)
-        units = max(L.add_suz_scratch(n), L.max_ruz_scratch(n), L.add_sez_scratch(n, self.m),
-                    L.pk2_luz_scratch(n), L.smp_puz_scratch(n), L.rku_luz_scratch(nr), 1)
-        self.scr = torch.empty(units * 8 + 4096, dtype=torch.uint8, device=dev)
+        self.scr = torch.empty(max(units) * 8 + 4096, dtype=torch.uint8, device=dev)
         self.max_result = None
         torch.cuda.synchronize()
 
@@ -150,6 +162,80 @@ def time_gpu(fn, reps: int, torch) -> float:
     e1.record(st)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+# ------------------------------------------------------------------ irregular float64 work
+def irregular(log2n: int, reps: int) -> dict:
+    """Segmented float64 +-scan and max-reduce over Zipf(1.5) segment lengths
+    clipped to [1, 2^20] (2^26 values uniform in [-1, 1]): every schedule
+    timed (median of `reps`, HIP events), the learned dispatcher's pick, the
+    exhaustive autotuner's best and the CVL entry points add_sed / max_red."""
+    import torch
+    from neslgpu_amd import dispatch as D
+    from neslgpu_amd.cvl import Cvl
+    cv = Cvl()
+    L = cv.lib
+    n = 1 << log2n
+    x = torch.empty(n, dtype=torch.float64, device=cv.device)
+    L.cvlhip_fill_splitmix_f64(x, n, SEED + 3, -1.0, 1.0)
+    lens = D.lengths("zipf", n, 1.5, np.random.default_rng(SEED + 3 & 0x7FFFFFFF), lmax=1 << 20)
+    sd = cv.segd(lens)
+    m = int(lens.size)
+    d = torch.empty_like(x)
+    r = torch.empty(m, dtype=torch.float64, device=cv.device)
+    tuner = D.Autotuner(cv, reps=reps)
+    disp = None
+    if os.path.exists(D.MODEL_PATH):
+        disp = D.Dispatcher(cv).load()
+    scr = cv.scratch(max(L.add_sed_scratch(n, m), L.max_red_scratch(n, m), L.cvlhip_seg_scratch(n, m)))
+    out = {"workload": f"2^{log2n} float64 uniform [-1,1], Zipf(1.5) segment lengths clipped to [1,2^20]",
+           "n": n, "segments": m, "arms": D.ARMS}
+    tot = {"learned": 0.0, "autotuned": 0.0, "cvl_entry": 0.0}
+    fixed = np.zeros(len(D.ARMS))
+    for kind, op, dst, byts in (("scan", "add", d, 16 * n + 4 * m), ("reduce", "max", r, 8 * n + 12 * m)):
+        t = tuner.tune(kind, op, "d", dst, x, sd)
+        fixed += t
+        best = int(np.argmin(t))
+        pick = disp.choose(D.features(cv, sd, kind, "d")) if disp is not None else None
+        if kind == "scan":
+            entry = lambda: L.add_sed(d, x, sd, n, m, scr)
+        else:
+            entry = lambda: L.max_red(r, x, sd, n, m, scr)
+        te = float(np.median([time_gpu(entry, 1, torch) for _ in range(reps)]))
+        rec = {"ms_per_arm": [round(float(v), 4) for v in t], "autotuned_arm": D.ARMS[best],
+               "autotuned_ms": round(float(t[best]), 4), "cvl_entry_ms": round(te, 4),
+               "compulsory_bytes": byts, "autotuned_GB/s": round(byts / t[best] / 1e6, 1)}
+        tot["autotuned"] += float(t[best])
+        tot["cvl_entry"] += te
+        if pick is not None:
+            rec["learned_arm"] = D.ARMS[pick]
+            rec["learned_ms"] = round(float(t[pick]), 4)
+            tot["learned"] += float(t[pick])
+        out[f"{op}_{kind}"] = rec
+    out["total_ms"] = {k: round(v, 4) for k, v in tot.items()}
+    out["fixed_arm_total_ms"] = {D.ARMS[a]: round(float(fixed[a]), 4) for a in range(len(D.ARMS))}
+    if disp is not None:
+        out["learned_regret_vs_autotuned"] = round(tot["learned"] / tot["autotuned"] - 1.0, 4)
+        out["learned_vs_best_fixed"] = round(tot["learned"] / float(fixed.min()) - 1.0, 4)
+    else:
+        out["learned"] = "no trained model (neslgpu_amd/dispatch_model.npz missing)"
+    return out
+
+
+def measured_traffic(primitive: str):
+    """HBM bytes per call of `primitive` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = t.get("primitives", {}).get(primitive)
+    if not rec:
+        return None
+    return rec.get("hbm_bytes")
 
 
 # ------------------------------------------------------------------ host baseline
@@ -243,10 +329,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    sw = Sweep(a.log2n, a.rank_log2n)
     names = ["add_suz", "max_ruz", "add_sez", "pk2_luz", "smp_puz", "rku_luz"]
     if a.only:
         names = [x for x in names + ["cpy_wuz"] if x in a.only.split(",")]
+    sw = Sweep(a.log2n, a.rank_log2n, names)
     byts = sw.bytes()
     for _ in range(a.warmup):
         sw.step(names)
@@ -276,7 +362,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": prims[dom]["frac_of_hbm_peak"],
-        "traffic": None,
+        "traffic": measured_traffic(dom),
+        "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
     }
     value = total_bytes / ms / 1e6
     out = {
@@ -301,6 +388,8 @@ def main():
         "primitives": prims,
         "roofline": roof,
     }
+    if not a.no_irregular and not a.only:
+        out["irregular_f64"] = irregular(a.irregular_log2n, max(3, a.steps))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

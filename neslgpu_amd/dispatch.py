@@ -13,7 +13,8 @@ passes and its Adam steps are HIP kernels of libcvl_hip.so
 changes with the choice, never the semantics; the autotuner stays the
 fallback and the judge of the dispatcher's regret.
 
-    python -m neslgpu_amd.dispatch train   # gather timings, train, save, evaluate
+    python -m neslgpu_amd.dispatch train [results.json] [model.npz]
+        # gather timings, train, save the model (default: next to this file), evaluate
 """
 from __future__ import annotations
 
@@ -182,10 +183,14 @@ class Dispatcher:
         return int(np.argmin(self.predict(feat[None, :])[0]))
 
     def save(self, path: str = MODEL_PATH):
-        np.savez(path, params=self.params.cpu().numpy())
+        np.savez(path, params=self.params.cpu().numpy(),
+                 dims=np.array([self.F, self.H, self.A, self.P], np.int64))
 
     def load(self, path: str = MODEL_PATH) -> "Dispatcher":
         with np.load(path, allow_pickle=False) as z:
+            if "dims" in z and tuple(int(v) for v in z["dims"]) != (self.F, self.H, self.A, self.P):
+                raise ValueError(f"{path}: network shape {tuple(z['dims'])} does not match the library's "
+                                 f"{(self.F, self.H, self.A, self.P)}")
             self.params = self._dev(z["params"])
         return self
 
@@ -248,7 +253,7 @@ def main(argv=None):
     Xtr = np.concatenate([X, X2]); Ttr = np.concatenate([T, T2]); Mtr = np.concatenate([M, M2])
     disp = Dispatcher(cv)
     losses = disp.train(Xtr, Ttr, Mtr, epochs=3000)
-    disp.save()
+    disp.save(argv[2] if len(argv) > 2 else MODEL_PATH)
     Xe, Te, Me, metae = gather(cv, EVAL_WORKLOADS, n_eval, seed=11)
     res = {"train_cases": int(len(Xtr)), "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),
            "eval": evaluate(disp, Xe, Te, metae), "eval_cases": metae, "seconds": round(time.time() - t0, 1)}
