@@ -1,4 +1,5 @@
-"""Builds neslgpu_amd/libcvl_hip.so, the gfx950 HIP backend of the CVL API.
+"""Builds neslgpu_amd/libcvl_hip.so, the gfx950 HIP backend of the CVL API,
+and neslgpu_amd/bin/vinterp, the VCODE interpreter that drives it.
 
     python -m neslgpu_amd.build          # compile every source, link
 
@@ -21,7 +22,11 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcvl_hip.so")
 ARCH = os.environ.get("CVLHIP_ARCH", "gfx950")
 
+VCODE = os.path.join(HERE, "vcode")
+VINTERP = os.path.join(HERE, "bin", "vinterp")
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,11 +44,29 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_vinterp(verbose: bool = True) -> str:
+    """The VCODE interpreter: host C++ only; it loads a CVL library with dlopen."""
+    os.makedirs(os.path.dirname(VINTERP), exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(VCODE, "*.cpp")))
+    tmp = VINTERP + ".tmp"
+    cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-o", tmp, *srcs, "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("building vinterp failed")
+    os.replace(tmp, VINTERP)
+    if verbose:
+        print(f"built {VINTERP}")
+    return VINTERP
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs) + 1)) as ex:
+        vint = ex.submit(build_vinterp, verbose)
         objs = list(ex.map(_compile, srcs))
+        vint.result()
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
