@@ -1,0 +1,180 @@
+"""Super-operator definitions: parser and type checker.
+
+A fused VCODE program (.fcode) replaces runs of elementwise instructions
+with `FUSED n`; super-op n is defined by one line of the form the reference
+fusion compiler writes (its generator is nesl/fusion/super-op-gen.sml:42-480;
+examples trail each bench/NESL/*.cu):
+
+    fused OP3#2 ($0 : INT, $1 : SEGDES) = (/ INT @ $0 (DIST INT @ (CONST INT 2) $1))
+
+The right-hand side is an expression tree over the arguments `$i`:
+`(OP [TYPE] @ e1 e2 ...)` applies VCODE elementwise op OP at TYPE,
+`(CONST TYPE v)` is a scalar, and `(DIST TYPE @ c $s)` spreads a scalar over
+the elements segment descriptor $s describes.  Segment-descriptor arguments
+only give the length.
+"""
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+
+TYPES = ("INT", "FLOAT", "BOOL", "SEGDES")
+
+
+@dataclass
+class Arg:
+    index: int
+
+
+@dataclass
+class Const:
+    type: str
+    value: str
+
+
+@dataclass
+class Apply:
+    op: str
+    type: str | None
+    args: list = field(default_factory=list)
+
+
+@dataclass
+class SuperOp:
+    number: int
+    uses: int
+    params: list            # types of $0, $1, ...
+    body: object
+    result: str = ""        # result type, filled by typecheck
+    text: str = ""
+
+
+# result type of each elementwise op, given its operand type
+_SAME = {"+", "-", "*", "/", "%", "NOT", "AND", "OR", "XOR", "MAX", "MIN"}
+_BOOL = {"<", "<=", ">", ">=", "=", "!="}
+_FIXED = {
+    "LSHIFT": "INT", "RSHIFT": "INT", "B_TO_I": "INT", "I_TO_F": "FLOAT", "I_TO_B": "BOOL",
+    "ROUND": "INT", "FLOOR": "INT", "CEIL": "INT", "TRUNC": "INT",
+    "LOG": "FLOAT", "SQRT": "FLOAT", "EXP": "FLOAT", "SIN": "FLOAT", "COS": "FLOAT", "TAN": "FLOAT",
+    "ASIN": "FLOAT", "ACOS": "FLOAT", "ATAN": "FLOAT", "SINH": "FLOAT", "COSH": "FLOAT", "TANH": "FLOAT",
+}
+# operand type of the ops whose instruction carries no type
+_OPERAND = {
+    "LSHIFT": "INT", "RSHIFT": "INT", "B_TO_I": "BOOL", "I_TO_F": "INT", "I_TO_B": "INT", "%": "INT",
+    "ROUND": "FLOAT", "FLOOR": "FLOAT", "CEIL": "FLOAT", "TRUNC": "FLOAT",
+    "LOG": "FLOAT", "SQRT": "FLOAT", "EXP": "FLOAT", "SIN": "FLOAT", "COS": "FLOAT", "TAN": "FLOAT",
+    "ASIN": "FLOAT", "ACOS": "FLOAT", "ATAN": "FLOAT", "SINH": "FLOAT", "COSH": "FLOAT", "TANH": "FLOAT",
+}
+
+_HEAD = re.compile(r"^fused\s+OP(\d+)#(\d+)\s*\((.*?)\)\s*=\s*(.*)$")
+
+
+class DefinitionError(ValueError):
+    pass
+
+
+def _tokens(s: str):
+    return re.findall(r"\(|\)|@|[^\s()@]+", s)
+
+
+def _parse_expr(toks, pos):
+    t = toks[pos]
+    if t.startswith("$"):
+        return Arg(int(t[1:])), pos + 1
+    if t != "(":
+        raise DefinitionError(f"unexpected token {t!r}")
+    op = toks[pos + 1]
+    pos += 2
+    if op == "CONST":
+        typ, val, close = toks[pos], toks[pos + 1], toks[pos + 2]
+        if close != ")" or typ not in TYPES:
+            raise DefinitionError("malformed CONST")
+        return Const(typ, val), pos + 3
+    typ = None
+    if toks[pos] in TYPES:
+        typ = toks[pos]
+        pos += 1
+    if toks[pos] != "@":
+        raise DefinitionError(f"expected @ after {op}")
+    pos += 1
+    args = []
+    while toks[pos] != ")":
+        e, pos = _parse_expr(toks, pos)
+        args.append(e)
+    return Apply(op, typ, args), pos + 1
+
+
+def parse_line(line: str) -> SuperOp:
+    m = _HEAD.match(line.strip())
+    if not m:
+        raise DefinitionError(f"not a super-op definition: {line!r}")
+    params = []
+    if m.group(3).strip():
+        for i, p in enumerate(m.group(3).split(",")):
+            name, typ = [x.strip() for x in p.split(":")]
+            if name != f"${i}" or typ not in TYPES:
+                raise DefinitionError(f"bad parameter {p!r}")
+            params.append(typ)
+    toks = _tokens(m.group(4))
+    body, pos = _parse_expr(toks, 0)
+    if pos != len(toks):
+        raise DefinitionError("trailing tokens")
+    op = SuperOp(int(m.group(1)), int(m.group(2)), params, body, text=line.strip())
+    op.result = typecheck(op)
+    return op
+
+
+def typecheck(op: SuperOp) -> str:
+    def ty(e):
+        if isinstance(e, Arg):
+            if e.index >= len(op.params):
+                raise DefinitionError(f"${e.index} is not a parameter")
+            t = op.params[e.index]
+            if t == "SEGDES":
+                raise DefinitionError("a segment descriptor used as a value")
+            return t
+        if isinstance(e, Const):
+            return e.type
+        if e.op == "DIST":
+            if len(e.args) != 2 or not isinstance(e.args[1], Arg) or op.params[e.args[1].index] != "SEGDES":
+                raise DefinitionError("DIST needs a scalar and a segment descriptor")
+            if not isinstance(e.args[0], Const):
+                raise DefinitionError("only constants can be distributed inside a fused op")
+            return ty(e.args[0])
+        if e.op == "SELECT":
+            f, a, b = (ty(x) for x in e.args)
+            if f != "BOOL" or a != b:
+                raise DefinitionError("SELECT types")
+            return a
+        if e.op == "RAND":
+            raise DefinitionError("RAND cannot be fused")
+        ts = [ty(x) for x in e.args]
+        want = e.type or _OPERAND.get(e.op)
+        if want is None:
+            raise DefinitionError(f"{e.op} needs a type")
+        if any(t != want for t in ts):
+            raise DefinitionError(f"{e.op} {want} applied to {ts}")
+        if e.op in _SAME:
+            return want
+        if e.op in _BOOL:
+            return "BOOL"
+        if e.op in _FIXED:
+            return _FIXED[e.op]
+        raise DefinitionError(f"unknown elementwise op {e.op}")
+    return ty(op.body)
+
+
+def parse(text: str) -> list:
+    """All definitions of a text (one may wrap over several lines), ordered
+    by number; numbers must be 0..n-1."""
+    defs = []
+    for l in text.splitlines():
+        if l.strip().startswith("fused "):
+            defs.append(l.strip())
+        elif defs and l.strip():
+            defs[-1] += " " + l.strip()
+    ops = [parse_line(d) for d in defs]
+    ops.sort(key=lambda o: o.number)
+    if [o.number for o in ops] != list(range(len(ops))):
+        raise DefinitionError("super-ops must be numbered 0..n-1 without gaps")
+    return ops

@@ -28,6 +28,9 @@
 #include "vinterp.hpp"
 
 #include <dlfcn.h>
+#include <cstddef>
+
+#include "../../include/vcode_fused.h"
 
 #include <array>
 #include <cerrno>
@@ -41,8 +44,14 @@
 
 namespace vcode {
 
-typedef void *vec_p;
-typedef int cvl_bool;
+// the interpreter's view of a fused library's tables is the public ABI
+static_assert(sizeof(VopDes) == sizeof(vopdes_t), "vopdes_t layout");
+static_assert(offsetof(VopDes, arg_type) == offsetof(vopdes_t, arg_type), "vopdes_t layout");
+static_assert(offsetof(VopDes, result_len) == offsetof(vopdes_t, result_len), "vopdes_t layout");
+static_assert(offsetof(VopDes, cvl_desc) == offsetof(vopdes_t, cvl_desc), "vopdes_t layout");
+static_assert(sizeof(CvlTriple) == sizeof(cvl_triple_t), "cvl_triple_t layout");
+static_assert(T_Segdes == (int)Segdes && T_Given == (int)Given, "TYPE values");
+static_assert(D_Scan == (int)Scan && D_NotSupported == (int)NotSupported, "cvl_desc_t values");
 
 [[noreturn]] static void die(const char *fmt, ...) {
     va_list ap;

@@ -85,6 +85,17 @@ def _functions(lines):
     return out
 
 
+def entry_size(text: str) -> int:
+    """The size constant (the first CONST INT) of the entry function."""
+    lines = text.splitlines()
+    name, start, end = _functions(lines)[0]
+    for i in range(start + 1, end):
+        m = re.match(r"^\s*CONST INT (-?\d+)\s*$", lines[i])
+        if m:
+            return int(m.group(1))
+    raise ValueError(f"no size constant in {name}")
+
+
 def set_size(text: str, n: int) -> str:
     """Replace the size constant (the first CONST INT) of the entry function."""
     lines = text.splitlines(keepends=True)

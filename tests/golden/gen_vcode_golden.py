@@ -77,6 +77,11 @@ VARIANTS = [
     ("qs.1048576.full", {"base": "qs.1000.vcode", "full": True, "size": 1 << 20,
                          "key_range": 2147483647, "digest": True}),
     ("quickhull.100000.full", {"base": "quickhull.1000.vcode", "full": True, "size": 100000, "digest": True}),
+    # fused programs print what their unfused forms print
+    ("qs.1000.fcode.full", {"base": "qs.1000.fcode", "full": True, "same_as": "qs.1000.full",
+                            "superops": "qs.1000.superops"}),
+    ("quickhull.1000.fcode.full", {"base": "quickhull.1000.fcode", "full": True,
+                                   "same_as": "quickhull.1000.full", "superops": "quickhull.1000.superops"}),
 ]
 
 ARENA_UNITS = "200000000"
@@ -101,6 +106,11 @@ def stable_digest(stdout, volatile):
     return hashlib.sha256("\n".join(lines).encode()).hexdigest()
 
 
+def masked(stdout, volatile):
+    """stdout with the run-dependent (timer) lines replaced by a marker."""
+    return "".join(("<timer>\n" if i in volatile else l + "\n") for i, l in enumerate(stdout.splitlines()))
+
+
 def run(prog):
     r = subprocess.run([VINTERP, "-B", SERIAL, "-m", ARENA_UNITS, prog], capture_output=True, text=True,
                        timeout=600)
@@ -118,9 +128,18 @@ def main():
     for name, rel in PROGRAMS:
         shutil.copyfile(os.path.join(REF, rel), os.path.join(PROGS, name))
     for name, rel in SUPEROPS:
-        lines = [l for l in open(os.path.join(REF, rel)) if l.startswith("fused OP")]
+        # a definition may wrap over several lines; it ends where the next
+        # one starts or the comment block closes
+        defs = []
+        for l in open(os.path.join(REF, rel)):
+            if l.startswith("fused OP"):
+                defs.append(l.strip())
+            elif defs and not l.startswith("*/") and defs[-1] is not None:
+                defs[-1] += " " + l.strip()
+            if l.startswith("*/") and defs:
+                defs.append(None)
         with open(os.path.join(OUT, "superops", name + ".superops"), "w") as f:
-            f.writelines(lines)
+            f.writelines(" ".join(d.split()) + "\n" for d in defs if d)
     manifest = {}
     for name, _ in PROGRAMS:
         if name.endswith(".fcode"):
@@ -136,7 +155,7 @@ def main():
                 (len(l1) - 1) not in volatile:
             volatile.append(len(l1) - 1)
         with open(os.path.join(OUT, name + ".out"), "w") as f:
-            f.write(out1)
+            f.write(masked(out1, volatile))
         with open(os.path.join(OUT, name + ".err"), "w") as f:
             f.write(err1)
         manifest[name] = {"volatile_stdout_lines": volatile, "stdout_lines": len(l1)}
@@ -155,16 +174,23 @@ def main():
             rec["sha256"] = stable_digest(out1, volatile)
         else:
             with open(os.path.join(OUT, name + ".out"), "w") as g:
-                g.write(out1)
+                g.write(masked(out1, volatile))
         manifest[name] = rec
         print(name, len(l1), "lines", flush=True)
     for name, spec in VARIANTS:
         if "same_as" in spec:
             manifest[name] = dict(manifest[spec["same_as"]], **spec)
+            manifest[name]["expected"] = spec["same_as"]
+    sys.path.insert(0, ROOT)
+    from neslgpu_amd import vinterp as V
     for name, _ in PROGRAMS:
         if name.endswith(".fcode"):
             base = name[: -len(".fcode")] + ".vcode"
-            manifest[name] = dict(manifest[base], expected=base, superops=name[: -len(".fcode")] + ".superops")
+            # a fused program runs at its unfused form's size (some were
+            # generated for another size than their file name says)
+            size = int(V.entry_size(V.read(os.path.join(PROGS, base))))
+            manifest[name] = dict(manifest[base], expected=base, superops=name[: -len(".fcode")] + ".superops",
+                                  base=name, size=size)
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
