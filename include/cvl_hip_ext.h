@@ -33,6 +33,22 @@ void cvlhip_synchronize(void);
  * resident blocks as the occupancy allows; -1 = one block per tile. */
 void cvlhip_set_blocks_per_cu(int cap);
 
+/* ---- HIP graphs ------------------------------------------------------
+ * A host can record a sequence of CVL calls that return no host data
+ * (vector operations, not reductions to a scalar, ext_vu*, v2c, ...) into
+ * a HIP graph and replay it: the VCODE interpreter does this for basic
+ * blocks (option -G).  Capture needs a stream of its own. */
+/* A new non-blocking stream (default priority), for cvlhip_set_stream. */
+void *cvlhip_stream_create(void);
+/* Start capturing the work queued on the current stream; 0 on success. */
+int cvlhip_capture_begin(void);
+/* End the capture and instantiate it; *exec receives the executable graph.
+ * Returns 0 on success; on failure nothing was executed. */
+int cvlhip_capture_end(void **exec);
+/* Replay an executable graph on the current stream; 0 on success. */
+int cvlhip_graph_launch(void *exec);
+void cvlhip_graph_destroy(void *exec);
+
 /* ---- random numbers -------------------------------------------------- */
 /* The 48-bit lrand48 state behind RAND (rnd_wuz).  It starts at 0, as an
  * unseeded glibc lrand48 does; rnd_foz leaves it alone, as the serial CVL

@@ -62,13 +62,15 @@ def build_vinterp(verbose: bool = True) -> str:
 
 FUSED_DIR = os.path.join(HERE, "fused")
 SUPEROPS_DIR = os.path.join(os.path.dirname(HERE), "tests", "golden", "vcode", "superops")
+PROGRAMS_DIR = os.path.join(HERE, "vcode", "programs")
 
 
 def build_fused(verbose: bool = True) -> list:
     """Fused-operation libraries for the super-op definitions the tests and
     the benchmark use (neslgpu_amd/fused/<program>.so)."""
     from .fusion import codegen
-    defs = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")))
+    defs = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")) +
+                  glob.glob(os.path.join(PROGRAMS_DIR, "*.superops")))
     outs = [os.path.join(FUSED_DIR, os.path.basename(d)[: -len(".superops")] + ".so") for d in defs]
     with ThreadPoolExecutor(max_workers=min(8, max(1, len(defs)))) as ex:
         list(ex.map(lambda a: codegen.build(open(a[0]).read(), a[1], verbose=verbose), zip(defs, outs)))

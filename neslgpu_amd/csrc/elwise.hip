@@ -17,6 +17,8 @@
 
 namespace cvlhip {
 
+static void *g_ext_host = nullptr;   // pinned landing slot of scalar extracts
+
 static int ew_grid(int64_t n) {
     int g = grid_for(n, kBlock * 4);
     return g > 16384 ? 16384 : g;
@@ -46,6 +48,12 @@ __global__ void k_sel(T *__restrict__ d, const int *__restrict__ f, const T *__r
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) d[i] = f[i] ? a[i] : b[i];
+}
+
+// one element written by one lane (scalar replace without a host wait)
+template <class T>
+__global__ void k_put1(T *p, T v) {
+    if (threadIdx.x == 0) *p = v;
 }
 
 template <class T>
@@ -332,15 +340,18 @@ void ind_lez(vec_p d, vec_p init, vec_p stride, vec_p dest_segd, int vec_len, in
 int ind_lez_scratch(int n, int m) { (void)n; (void)m; return 0; }
 unsigned int ind_lez_inplace(void) { return INPLACE_NONE; }
 
-// ---- extract / replace of one element: a host <-> device copy
+// ---- extract / replace of one element
+// EXT reads the element into pinned host memory and waits for it (the caller
+// needs the value); REP writes it with a one-lane kernel that takes the value
+// as an argument, so it queues behind earlier work without a host wait.
 #define EXT(NAME, T)                                                                           \
     T NAME(vec_p V, int i, int len, vec_p scratch) {                                           \
         (void)scratch;                                                                         \
         if (i < 0 || i >= len) { usage_error(#NAME ": index %d outside [0,%d)", i, len); return (T)0; } \
-        T h;                                                                                   \
-        HIPCHECK(hipMemcpyAsync(&h, (const T *)V + i, sizeof(T), hipMemcpyDeviceToHost, g_stream)); \
+        if (!g_ext_host) HIPCHECK(hipHostMalloc(&g_ext_host, 64, hipHostMallocDefault));      \
+        HIPCHECK(hipMemcpyAsync(g_ext_host, (const T *)V + i, sizeof(T), hipMemcpyDeviceToHost, g_stream)); \
         HIPCHECK(hipStreamSynchronize(g_stream));                                              \
-        return h;                                                                              \
+        return *(const T *)g_ext_host;                                                         \
     }                                                                                          \
     NOSCR(NAME, INPLACE_NONE)
 EXT(ext_vuz, int)
@@ -351,9 +362,8 @@ EXT(ext_vud, double)
     void NAME(vec_p V, int i, T val, int len, vec_p scratch) {                                 \
         (void)scratch;                                                                         \
         if (i < 0 || i >= len) { usage_error(#NAME ": index %d outside [0,%d)", i, len); return; } \
-        T h = NORM(val);                                                                       \
-        HIPCHECK(hipMemcpyAsync((T *)V + i, &h, sizeof(T), hipMemcpyHostToDevice, g_stream));  \
-        HIPCHECK(hipStreamSynchronize(g_stream));                                              \
+        k_put1<T><<<1, 64, 0, g_stream>>>((T *)V + i, (T)NORM(val));                          \
+        LAUNCHCHECK();                                                                         \
     }                                                                                          \
     NOSCR(NAME, INPLACE_NONE)
 #define IDENT(x) (x)

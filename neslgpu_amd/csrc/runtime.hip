@@ -158,6 +158,44 @@ unsigned cvlhip_check_device_errors(void) {
 
 void cvlhip_synchronize(void) { HIPCHECK(hipStreamSynchronize(g_stream)); }
 
+// ---- HIP graphs of CVL call sequences
+void *cvlhip_stream_create(void) {
+    hipStream_t s = nullptr;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return (void *)s;
+}
+
+int cvlhip_capture_begin(void) {
+    if (g_stream == nullptr) return -1;                // the null stream cannot be captured
+    return hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : -1;
+}
+
+int cvlhip_capture_end(void **exec) {
+    hipGraph_t g = nullptr;
+    *exec = nullptr;
+    if (hipStreamEndCapture(g_stream, &g) != hipSuccess || g == nullptr) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    hipGraphExec_t e = nullptr;
+    hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    *exec = (void *)e;
+    return 0;
+}
+
+int cvlhip_graph_launch(void *exec) {
+    return hipGraphLaunch((hipGraphExec_t)exec, g_stream) == hipSuccess ? 0 : -1;
+}
+
+void cvlhip_graph_destroy(void *exec) {
+    if (exec) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
 void cvlhip_set_blocks_per_cu(int cap) { g_blocks_per_cu_cap = cap; }
 
 // ------------------------------------------------------------------ CVL facilities

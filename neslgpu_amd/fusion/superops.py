@@ -12,6 +12,12 @@ The right-hand side is an expression tree over the arguments `$i`:
 `(CONST TYPE v)` is a scalar, and `(DIST TYPE @ c $s)` spreads a scalar over
 the elements segment descriptor $s describes.  Segment-descriptor arguments
 only give the length.
+
+One form goes beyond the reference fusion compiler, which fuses elementwise
+code only: `(GATHER TYPE @ $src e $s)` reads $src at the index e evaluates to
+(an unsegmented BPERMUTE folded into the fused op; $s is the one-segment
+descriptor of $src, whose length bounds the index: an index outside
+[0, length) gives 0).  $src need not have the length of the other arguments.
 """
 from __future__ import annotations
 
@@ -141,6 +147,15 @@ def typecheck(op: SuperOp) -> str:
             if not isinstance(e.args[0], Const):
                 raise DefinitionError("only constants can be distributed inside a fused op")
             return ty(e.args[0])
+        if e.op == "GATHER":
+            if len(e.args) != 3 or not isinstance(e.args[0], Arg) or not isinstance(e.args[2], Arg):
+                raise DefinitionError("GATHER needs $src, an index and $s")
+            if op.params[e.args[2].index] != "SEGDES" or ty(e.args[1]) != "INT":
+                raise DefinitionError("GATHER index must be INT and $s a segment descriptor")
+            t = ty(e.args[0])
+            if e.type and e.type != t:
+                raise DefinitionError("GATHER type")
+            return t
         if e.op == "SELECT":
             f, a, b = (ty(x) for x in e.args)
             if f != "BOOL" or a != b:
@@ -162,6 +177,17 @@ def typecheck(op: SuperOp) -> str:
             return _FIXED[e.op]
         raise DefinitionError(f"unknown elementwise op {e.op}")
     return ty(op.body)
+
+
+def gather_sources(e, acc=None):
+    """{src index: segd index} of the GATHER forms of an expression."""
+    acc = {} if acc is None else acc
+    if isinstance(e, Apply):
+        if e.op == "GATHER":
+            acc[e.args[0].index] = e.args[2].index
+        for x in e.args:
+            gather_sources(x, acc)
+    return acc
 
 
 def parse(text: str) -> list:

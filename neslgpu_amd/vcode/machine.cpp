@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <functional>
 #include <limits>
 #include <set>
 #include <utility>
@@ -279,6 +280,7 @@ class Machine {
         if (!base) die("cannot allocate a vector arena of %lld units", units);
         arena_.init(static_cast<char *>(base), (size_t)units);
         if (prog_.uses_fused) load_fused();
+        if (opt_.graphs) setup_graphs();
         install_constants();
 
         auto it = prog_.labels.find(prog_.entry);
@@ -289,6 +291,11 @@ class Machine {
         if (opt_.time_run) {
             be_.tgt_fos(t1);
             std::fprintf(stderr, "elapsed time = %f seconds\n", be_.tdf_fos(t1, t0));
+            std::fprintf(stderr, "instructions executed = %llu, peak vector memory = %zu units\n",
+                         executed_, arena_.peak());
+            if (graphs_)
+                std::fprintf(stderr, "graphs: %llu basic blocks captured, %llu replayed\n", graph_captures_,
+                             graph_replays_);
         }
         for (Vec *v : st_) {
             if (v->type == T_Pair)
@@ -301,6 +308,119 @@ class Machine {
     }
 
   private:
+    // ---------------------------------------------------------- device calls
+    // Every CVL call that queues device work without returning host data goes
+    // through dev(): executed at once, or, while a basic block is being
+    // recorded for a HIP graph, kept (with its arguments as the graph key).
+    struct Recorder {
+        bool on = false;
+        std::vector<std::function<void()>> calls;
+        std::vector<uint64_t> key;
+    };
+    struct GraphEntry {
+        std::vector<uint64_t> key;
+        void *exec = nullptr;
+    };
+
+    void key_push(const void *p) { rec_.key.push_back((uint64_t)(uintptr_t)p); }
+    void key_push(int v) { rec_.key.push_back((uint64_t)(uint32_t)v); }
+    void key_push(double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, 8);
+        rec_.key.push_back(u);
+    }
+
+    template <class F, class... A>
+    void dev(F f, A... a) {
+        if (!rec_.on) {
+            f(a...);
+            return;
+        }
+        key_push(reinterpret_cast<const void *>(f));
+        (key_push(a), ...);
+        rec_.calls.push_back([=]() { f(a...); });
+    }
+
+    void setup_graphs() {
+        using StreamCreate = void *(*)();
+        using SetStream = void (*)(void *);
+        auto create = reinterpret_cast<StreamCreate>(be_.sym("cvlhip_stream_create"));
+        auto set = reinterpret_cast<SetStream>(be_.sym("cvlhip_set_stream"));
+        cap_begin_ = reinterpret_cast<int (*)()>(be_.sym("cvlhip_capture_begin"));
+        cap_end_ = reinterpret_cast<int (*)(void **)>(be_.sym("cvlhip_capture_end"));
+        graph_launch_ = reinterpret_cast<int (*)(void *)>(be_.sym("cvlhip_graph_launch"));
+        graph_destroy_ = reinterpret_cast<void (*)(void *)>(be_.sym("cvlhip_graph_destroy"));
+        if (!create || !set || !cap_begin_ || !cap_end_ || !graph_launch_ || !graph_destroy_) {
+            std::fprintf(stderr, "vinterp: the CVL library cannot capture graphs; -G ignored\n");
+            return;
+        }
+        set(create());
+        graphs_ = true;
+    }
+
+    // Which instructions a graph may hold: stack manipulation, constants and
+    // vector operations that only queue device work.  Control flow, anything
+    // that returns data to the host (MAKE_SEGDES and PACK need a length, IF a
+    // flag, I/O), RAND (it advances a host-side generator state) and rank
+    // (it inspects key histograms on the host) end a block.
+    bool capturable(int pc, const Instr &in) {
+        switch (in.op) {
+            case OP_COPY: case OP_POP: case OP_CPOP: case OP_PAIR: case OP_UNPAIR: case OP_CONST:
+            case OP_LENGTH: case OP_LENGTHS:
+                return true;
+            case OP_RAND: case OP_RANK_UP: case OP_RANK_DOWN:
+                return false;
+            case OP_FUSED: {
+                const Sig &s = signature(pc, in);
+                return s.desc >= D_Elwise0 && s.desc <= D_Elwise40;
+            }
+            default: break;
+        }
+        const int d = op_info(in.op).desc;
+        return (d >= D_Elwise0 && d <= D_Elwise40) || d == D_Scan || d == D_Reduce || d == D_Permute ||
+               d == D_Dpermute || d == D_Fpermute || d == D_Bpermute || d == D_Bfpermute || d == D_Dfpermute ||
+               d == D_Extract || d == D_Replace || d == D_Dist || d == D_Index;
+    }
+
+    void run_recorded() {
+        for (auto &c : rec_.calls) c();
+    }
+
+    // end of a recorded block: replay its graph, or capture one
+    void flush() {
+        rec_.on = false;
+        if (rec_.calls.size() < 2) {
+            run_recorded();
+            return;
+        }
+        uint64_t h = 1469598103934665603ull;
+        for (uint64_t w : rec_.key) h = (h ^ w) * 1099511628211ull;
+        auto it = graphs_cache_.find(h);
+        if (it != graphs_cache_.end() && it->second.key == rec_.key) {
+            if (graph_launch_(it->second.exec) != 0) die("graph launch failed");
+            ++graph_replays_;
+            return;
+        }
+        if (cap_begin_() != 0) {
+            run_recorded();
+            return;
+        }
+        run_recorded();
+        void *exec = nullptr;
+        if (cap_end_(&exec) != 0) die("capturing a basic block failed");
+        if (graphs_cache_.size() >= 512) {
+            graph_destroy_(graphs_cache_.begin()->second.exec);
+            graphs_cache_.erase(graphs_cache_.begin());
+        }
+        if (it != graphs_cache_.end()) {          // hash collision: keep the newer block
+            graph_destroy_(it->second.exec);
+            graphs_cache_.erase(it);
+        }
+        graphs_cache_[h] = GraphEntry{rec_.key, exec};
+        if (graph_launch_(exec) != 0) die("graph launch failed");
+        ++graph_captures_;
+    }
+
     // ---------------------------------------------------------- memory
     int units_of(int type, int len, int seg_len) {
         switch (type) {
@@ -370,19 +490,19 @@ class Machine {
     void push_int_scalar(int v) {
         Vec *r = alloc(T_Int, 1);
         Scratch s(*this, be_.rep_vuz_scratch(1));
-        be_.rep_vuz(r->p, 0, v, 1, s.p);
+        dev(be_.rep_vuz, r->p, 0, v, 1, s.p);
         push(r);
     }
     void push_bool_scalar(int v) {
         Vec *r = alloc(T_Bool, 1);
         Scratch s(*this, be_.rep_vub_scratch(1));
-        be_.rep_vub(r->p, 0, v ? 1 : 0, 1, s.p);
+        dev(be_.rep_vub, r->p, 0, v ? 1 : 0, 1, s.p);
         push(r);
     }
     void push_float_scalar(double v) {
         Vec *r = alloc(T_Float, 1);
         Scratch s(*this, be_.rep_vud_scratch(1));
-        be_.rep_vud(r->p, 0, v, 1, s.p);
+        dev(be_.rep_vud, r->p, 0, v, 1, s.p);
         push(r);
     }
     int int_scalar(Vec *v, const char *what) {
@@ -635,9 +755,9 @@ class Machine {
 
     void copy_vector(Vec *d, Vec *s) {
         switch (s->type) {
-            case T_Int: { Scratch x(*this, be_.cpy_wuz_scratch(s->len)); be_.cpy_wuz(d->p, s->p, s->len, x.p); break; }
-            case T_Bool: { Scratch x(*this, be_.cpy_wub_scratch(s->len)); be_.cpy_wub(d->p, s->p, s->len, x.p); break; }
-            case T_Float: { Scratch x(*this, be_.cpy_wud_scratch(s->len)); be_.cpy_wud(d->p, s->p, s->len, x.p); break; }
+            case T_Int: { Scratch x(*this, be_.cpy_wuz_scratch(s->len)); dev(be_.cpy_wuz, d->p, s->p, s->len, x.p); break; }
+            case T_Bool: { Scratch x(*this, be_.cpy_wub_scratch(s->len)); dev(be_.cpy_wub, d->p, s->p, s->len, x.p); break; }
+            case T_Float: { Scratch x(*this, be_.cpy_wud_scratch(s->len)); dev(be_.cpy_wud, d->p, s->p, s->len, x.p); break; }
             default: die("line %d: illegal operand for REPLACE", line_);
         }
     }
@@ -654,41 +774,52 @@ class Machine {
             int len;
             if (k == 0) len = d->type == T_Segdes ? d->seg_len : d->len;
             else len = a[0]->type == T_Segdes ? a[0]->seg_len : a[0]->len;
-            vec_p src[kMaxArgs];
+            std::array<vec_p, kMaxArgs> src{};
             for (int j = 0; j < k; ++j) src[j] = a[j]->p;
-            kElwise[k](s.fn, d->p, src, len, scr);
+            if (!rec_.on) {
+                kElwise[k](s.fn, d->p, src.data(), len, scr);
+                return;
+            }
+            key_push(s.fn);
+            key_push(d->p);
+            for (int j = 0; j < k; ++j) key_push(src[j]);
+            key_push(len);
+            key_push(scr);
+            void *fn = s.fn;
+            vec_p dp = d->p;
+            rec_.calls.push_back([=]() mutable { kElwise[k](fn, dp, src.data(), len, scr); });
             return;
         }
         switch (s.desc) {
             case D_Scan: case D_Reduce:
-                reinterpret_cast<F6>(s.fn)(d->p, a[0]->p, a[1]->p, a[1]->seg_len, a[1]->len, scr);
+                dev(reinterpret_cast<F6>(s.fn), d->p, a[0]->p, a[1]->p, a[1]->seg_len, a[1]->len, scr);
                 return;
             case D_Permute:
-                reinterpret_cast<F7>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
+                dev(reinterpret_cast<F7>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
                 return;
             case D_Bpermute:
-                reinterpret_cast<F10>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, a[3]->p,
+                dev(reinterpret_cast<F10>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, a[3]->p,
                                             a[3]->seg_len, a[3]->len, scr);
                 return;
             case D_Fpermute: case D_Bfpermute: case D_Dpermute:
-                reinterpret_cast<F11>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[3]->p, a[3]->seg_len, a[3]->len,
+                dev(reinterpret_cast<F11>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[3]->p, a[3]->seg_len, a[3]->len,
                                             a[4]->p, a[4]->seg_len, a[4]->len, scr);
                 return;
             case D_Dfpermute:
-                reinterpret_cast<F12>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[3]->p, a[4]->p, a[4]->seg_len,
+                dev(reinterpret_cast<F12>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[3]->p, a[4]->p, a[4]->seg_len,
                                             a[4]->len, a[5]->p, a[5]->seg_len, a[5]->len, scr);
                 return;
             case D_Extract:
-                reinterpret_cast<F7>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
+                dev(reinterpret_cast<F7>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
                 return;
             case D_Replace:
-                reinterpret_cast<F7>(s.fn)(d->p, a[1]->p, a[2]->p, a[3]->p, a[3]->seg_len, a[3]->len, scr);
+                dev(reinterpret_cast<F7>(s.fn), d->p, a[1]->p, a[2]->p, a[3]->p, a[3]->seg_len, a[3]->len, scr);
                 return;
             case D_Dist:
-                reinterpret_cast<F6>(s.fn)(d->p, a[0]->p, a[1]->p, a[1]->seg_len, a[1]->len, scr);
+                dev(reinterpret_cast<F6>(s.fn), d->p, a[0]->p, a[1]->p, a[1]->seg_len, a[1]->len, scr);
                 return;
             case D_Index:
-                reinterpret_cast<F7>(s.fn)(d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
+                dev(reinterpret_cast<F7>(s.fn), d->p, a[0]->p, a[1]->p, a[2]->p, a[2]->seg_len, a[2]->len, scr);
                 return;
             default:
                 die("line %d: operation %s not supported", line_, s.what.c_str());
@@ -714,7 +845,7 @@ class Machine {
                 Vec *r = alloc(T_Int, sd->len);
                 {
                     Scratch s(*this, be_.len_fos_scratch(sd->seg_len, sd->len));
-                    be_.len_fos(r->p, sd->p, sd->seg_len, sd->len, s.p);
+                    dev(be_.len_fos, r->p, sd->p, sd->seg_len, sd->len, s.p);
                 }
                 pop_top();
                 push(r);
@@ -753,7 +884,7 @@ class Machine {
             typedef int (*S4)(int, int, int, int);
             typedef void (*F10)(vec_p, vec_p, vec_p, vec_p, int, int, vec_p, int, int, vec_p);
             Scratch x(*this, reinterpret_cast<S4>(s.scratch_fn)(sd->seg_len, sd->len, dsd->seg_len, dsd->len));
-            reinterpret_cast<F10>(s.fn)(dst->p, src->p, flags->p, sd->p, sd->seg_len, sd->len, dsd->p,
+            dev(reinterpret_cast<F10>(s.fn), dst->p, src->p, flags->p, sd->p, sd->seg_len, sd->len, dsd->p,
                                         dsd->seg_len, dsd->len, x.p);
         }
         for (int k = 0; k < 3; ++k) unref(args[k]);
@@ -1090,6 +1221,17 @@ class Machine {
         while (pc >= 0) {
             const Instr &in = prog_.code[pc];
             line_ = in.line;
+            ++executed_;
+            if (graphs_) {
+                const bool cap = capturable(pc, in);
+                if (cap && !rec_.on) {
+                    rec_.on = true;
+                    rec_.calls.clear();
+                    rec_.key.clear();
+                } else if (!cap && rec_.on) {
+                    flush();
+                }
+            }
             if (opt_.trace) std::fprintf(stderr, "%d: %s %s (stack %zu)\n", in.line, op_info(in.op).name,
                                          type_name(in.type), st_.size());
             int next = pc + 1;
@@ -1156,6 +1298,7 @@ class Machine {
                     }
                     break;
                 case OP_EXIT:
+                    if (rec_.on) flush();
                     std::fflush(stdout);
                     std::exit(0);
                 case OP_START_TIMER: be_.tgt_fos(t_start); break;
@@ -1182,6 +1325,7 @@ class Machine {
             }
             pc = next;
         }
+        if (rec_.on) flush();
     }
 
     const Program &prog_;
@@ -1196,6 +1340,15 @@ class Machine {
     const CvlTriple *funs_ = nullptr;
     int nfused_ = -1;
     int line_ = 0;
+    unsigned long long executed_ = 0;
+    Recorder rec_;
+    bool graphs_ = false;
+    std::map<uint64_t, GraphEntry> graphs_cache_;
+    unsigned long long graph_captures_ = 0, graph_replays_ = 0;
+    int (*cap_begin_)() = nullptr;
+    int (*cap_end_)(void **) = nullptr;
+    int (*graph_launch_)(void *) = nullptr;
+    void (*graph_destroy_)(void *) = nullptr;
     bool io_error_ = false;
     std::string io_msg_;
 };

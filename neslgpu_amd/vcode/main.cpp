@@ -1,13 +1,15 @@
 // main.cpp -- command line of the VCODE interpreter.
 //
-//   vinterp [-t] [-m units] [-l fused.so] [-B cvl.so] [-d command] [-P] file.vcode
+//   vinterp [-t] [-G] [-m units] [-l fused.so] [-B cvl.so] [-d command] [-P] file.vcode
 //
 // Options as in the reference interpreter (nesl/vcode/main.c:138-160):
 // -t times the run, -m sizes the vector arena in CVL units (8 bytes),
 // -l names the fused-operation library that FUSED n instructions call.
 // -B names the CVL backend (default: libcvl_hip.so next to this binary,
 // or $VINTERP_CVL).  -P only parses and links the program and prints its
-// instruction count.
+// instruction count.  -G records each basic block of device-only
+// instructions and replays it as a HIP graph when the same block runs again
+// on the same vectors (a loop body, an iterated kernel).
 #include <unistd.h>
 
 #include <cstdio>
@@ -20,7 +22,7 @@
 #include "vinterp.hpp"
 
 static void usage() {
-    std::fprintf(stderr, "vinterp [-t] [-P] [-m <units>] [-l <fused library>] [-B <CVL library>] [-d command] file-name\n");
+    std::fprintf(stderr, "vinterp [-t] [-G] [-P] [-m <units>] [-l <fused library>] [-B <CVL library>] [-d command] file-name\n");
     std::exit(1);
 }
 
@@ -43,7 +45,7 @@ int main(int argc, char **argv) {
     vcode::Options opt;
     int c;
     bool parse_only = false;
-    while ((c = getopt(argc, argv, "tm:l:B:d:cP")) != -1) {
+    while ((c = getopt(argc, argv, "tm:l:B:d:cPG")) != -1) {
         switch (c) {
             case 't': opt.time_run = true; break;
             case 'm': opt.arena_units = std::atoll(optarg); break;
@@ -52,6 +54,7 @@ int main(int argc, char **argv) {
             case 'd': if (std::strcmp(optarg, "command") == 0) opt.trace = true; break;
             case 'c': break;                       // arguments are always checked
             case 'P': parse_only = true; break;
+            case 'G': opt.graphs = true; break;
             default: usage();
         }
     }

@@ -157,6 +157,7 @@ struct Options {
     long long arena_units = 0;   // -m (0: default)
     bool time_run = false;       // -t
     bool trace = false;          // -d command
+    bool graphs = false;         // -G: replay repeated basic blocks as HIP graphs
 };
 
 int run(const Program &prog, const Options &opt);

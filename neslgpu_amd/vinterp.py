@@ -38,7 +38,8 @@ class Result:
 
 
 def run(program: str, *, backend: str | None = None, fused: str | None = None, arena_units: int | None = None,
-        timeout: float | None = None, timed: bool = False, check: bool = True) -> Result:
+        timeout: float | None = None, timed: bool = False, graphs: bool = False, check: bool = True,
+        stdin: str | None = None) -> Result:
     """Run a VCODE program given as a path or as text."""
     if not os.path.exists(VINTERP):
         raise FileNotFoundError(f"{VINTERP} not built (python -m neslgpu_amd.build)")
@@ -55,9 +56,11 @@ def run(program: str, *, backend: str | None = None, fused: str | None = None, a
         cmd += ["-m", str(int(arena_units))]
     if timed:
         cmd += ["-t"]
+    if graphs:
+        cmd += ["-G"]
     cmd.append(path)
     try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, input=stdin)
     finally:
         if tmp is not None:
             os.unlink(tmp.name)

@@ -84,6 +84,14 @@ VARIANTS = [
                                    "same_as": "quickhull.1000.full", "superops": "quickhull.1000.superops"}),
 ]
 
+# this package's own programs (neslgpu_amd/vcode/programs), with their stdin
+OWN = os.path.join(ROOT, "neslgpu_amd", "vcode", "programs")
+SPMV_ROWS = [3, 0, 2, 1, 17, 1, 1, 64, 0, 5] * 30 + [300, 2, 7]
+OWN_VARIANTS = [
+    ("spmv.small.full", {"own": "spmv.vcode", "full": True,
+                         "stdin": "( " + " ".join(map(str, SPMV_ROWS)) + " )\n( 257 )\n"}),
+]
+
 ARENA_UNITS = "200000000"
 
 
@@ -111,9 +119,9 @@ def masked(stdout, volatile):
     return "".join(("<timer>\n" if i in volatile else l + "\n") for i, l in enumerate(stdout.splitlines()))
 
 
-def run(prog):
+def run(prog, stdin=None):
     r = subprocess.run([VINTERP, "-B", SERIAL, "-m", ARENA_UNITS, prog], capture_output=True, text=True,
-                       timeout=600)
+                       timeout=600, input=stdin)
     if r.returncode != 0:
         raise SystemExit(f"{prog}: exit {r.returncode}\n{r.stderr[-2000:]}")
     err = "".join(l for l in r.stderr.splitlines(keepends=True) if not l.startswith("rnd_wuz:"))
@@ -121,6 +129,8 @@ def run(prog):
 
 
 def main():
+    sys.path.insert(0, ROOT)
+    from neslgpu_amd import vinterp as V
     if not os.path.exists(SERIAL):
         raise SystemExit("build the reference serial CVL first: make -C oracle")
     os.makedirs(PROGS, exist_ok=True)
@@ -177,12 +187,24 @@ def main():
                 g.write(masked(out1, volatile))
         manifest[name] = rec
         print(name, len(l1), "lines", flush=True)
+    for name, spec in OWN_VARIANTS:
+        text = open(os.path.join(OWN, spec["own"])).read()
+        if spec.get("full"):
+            text = V.full_result(text)
+        with tempfile.NamedTemporaryFile("w", suffix=".vcode", delete=False) as f:
+            f.write(text)
+        out1, _ = run(f.name, spec["stdin"])
+        os.unlink(f.name)
+        l1 = out1.splitlines()
+        volatile = [len(l1) - 1]
+        with open(os.path.join(OUT, name + ".out"), "w") as g:
+            g.write(masked(out1, volatile))
+        manifest[name] = dict(spec, volatile_stdout_lines=volatile, stdout_lines=len(l1))
+        print(name, len(l1), "lines", flush=True)
     for name, spec in VARIANTS:
         if "same_as" in spec:
             manifest[name] = dict(manifest[spec["same_as"]], **spec)
             manifest[name]["expected"] = spec["same_as"]
-    sys.path.insert(0, ROOT)
-    from neslgpu_amd import vinterp as V
     for name, _ in PROGRAMS:
         if name.endswith(".fcode"):
             base = name[: -len(".fcode")] + ".vcode"
