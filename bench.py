@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--no-irregular", action="store_true",
                    help="skip the irregular float64 segmented workload and its dispatch comparison")
     p.add_argument("--irregular-log2n", type=int, default=26)
+    p.add_argument("--no-programs", action="store_true",
+                   help="skip the whole-program workloads (quicksort, quickhull, SpMV through vinterp)")
     return p.parse_args()
 
 
@@ -222,6 +224,116 @@ def irregular(log2n: int, reps: int) -> dict:
     return out
 
 
+# ------------------------------------------------------------------ whole programs
+PROGRAMS = os.path.join(ROOT, "tests", "golden", "vcode", "programs")
+OWN_PROGRAMS = os.path.join(ROOT, "neslgpu_amd", "vcode", "programs")
+
+
+def power_law_rows(m: int, nnz: int, mean: float, lmax: int, seed: int) -> np.ndarray:
+    """m row lengths in [0, lmax] from a discrete power law with the given
+    mean, adjusted by +-1 steps so they sum to exactly nnz."""
+    L = np.arange(1, lmax + 1, dtype=np.float64)
+    lo, hi = 1.01, 4.0
+    for _ in range(60):                       # exponent giving the mean
+        a = 0.5 * (lo + hi)
+        p = L ** -a
+        if (p * L).sum() / p.sum() > mean:
+            lo = a
+        else:
+            hi = a
+    p = L ** -a
+    rng = np.random.default_rng(seed)
+    rows = rng.choice(np.arange(1, lmax + 1), size=m, p=p / p.sum()).astype(np.int64)
+    diff = nnz - int(rows.sum())
+    while diff != 0:
+        k = min(abs(diff), m)
+        idx = rng.choice(m, size=k, replace=False)
+        if diff > 0:
+            idx = idx[rows[idx] < lmax]
+            rows[idx] += 1
+            diff -= idx.size
+        else:
+            idx = idx[rows[idx] > 0]
+            rows[idx] -= 1
+            diff += idx.size
+    return rows.astype(np.int32)
+
+
+def program_workloads():
+    """(name, program text, stdin, fused library or None, graphs, repeats,
+    description) of the whole-program workloads."""
+    from neslgpu_amd import vinterp as V
+    from neslgpu_amd.build import FUSED_DIR
+    qs = V.set_function_int_const(V.set_size(V.read(os.path.join(PROGRAMS, "qs.1000.vcode")), 1 << 20),
+                                  "QS_4", 100, 2147483647)
+    qh = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.vcode")), 1 << 24)
+    qhf = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.fcode")), 1 << 24)
+    rows = power_law_rows(1 << 22, 1 << 26, 16.0, 1 << 16, SEED + 4 & 0x7FFFFFFF)
+    spmv_in = "( " + " ".join(map(str, rows.tolist())) + " )\n( " + str(1 << 22) + " )\n"
+    spmv = V.read(os.path.join(OWN_PROGRAMS, "spmv.vcode"))
+    spmvf = V.read(os.path.join(OWN_PROGRAMS, "spmv.fcode"))
+    return [
+        ("quicksort", qs, None, None, False, 1, "qs.nesl: 2^20 keys uniform in [0, 2^31-1) (RAND), sorted"),
+        ("quickhull", qh, None, None, False, 1, "quickhull.nesl: 2^24 points uniform in the unit disk"),
+        ("quickhull_fused", qhf, None, os.path.join(FUSED_DIR, "quickhull.1000.so"), False, 1,
+         "same, FUSED super-ops (generated HIP)"),
+        ("spmv", spmv, spmv_in, None, False, 10, "y = A x, CSR 2^22 rows, power-law lengths (mean 16, "
+         "max 2^16), 2^26 nnz, float64; BPERMUTE, *, +_REDUCE"),
+        ("spmv_fused", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), False, 10,
+         "same, gather and multiply fused in one kernel"),
+        ("spmv_fused_graph", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), True, 10,
+         "same, fused block replayed as a HIP graph"),
+    ], rows
+
+
+def program_timer(stdout: str) -> float:
+    return float(stdout.splitlines()[-1].strip("() "))
+
+
+def programs(reps: int) -> dict:
+    """Whole VCODE programs through vinterp on libcvl_hip.so: median of the
+    programs' own START_TIMER / STOP_TIMER interval (data generation
+    excluded), in ms.  SpMV repeats the product 10 times inside the timed
+    interval; its figures are per product."""
+    from neslgpu_amd import vinterp as V
+    work, rows = program_workloads()
+    out = {}
+    nnz, m, ncols = int(rows.sum()), int(rows.size), 1 << 22
+    for name, text, stdin, fused, graphs, k, desc in work:
+        ts, first = [], None
+        for _ in range(reps):
+            r = V.run(text, fused=fused, graphs=graphs, stdin=stdin, arena_units=(1 << 31) - 1, timeout=300)
+            ts.append(program_timer(r.stdout) / k * 1e3)
+            first = first or r.stdout.splitlines()[:-1]
+        rec = {"ms": round(float(np.median(ts)), 4), "runs_ms": [round(t, 4) for t in ts], "workload": desc}
+        if name.startswith("spmv"):
+            byts = nnz * 12 + ncols * 8 + m * (4 + 8)
+            rec["ms_per_product"] = rec.pop("ms")
+            rec["compulsory_bytes"] = byts
+            rec["GB/s"] = round(byts / (rec["ms_per_product"] * 1e-3) / 1e9, 1)
+            rec["sum_y"] = first[-1] if first else None
+        out[name] = rec
+    return out
+
+
+def cpu_programs() -> dict:
+    """The same programs through vinterp on the serial reference CVL
+    (oracle/_ref), single thread: one run each, ms."""
+    from neslgpu_amd import vinterp as V
+    from oracle import refcvl
+    if not refcvl.available():
+        return {"skipped": "reference serial CVL not built"}
+    serial = refcvl.LIB_PATH
+    work, _ = program_workloads()
+    out = {}
+    for name, text, stdin, fused, graphs, k, desc in work:
+        if fused:
+            continue                              # fused libraries are HIP code
+        r = V.run(text, backend=serial, stdin=stdin, arena_units=(1 << 31) - 1, timeout=600)
+        out[name] = {"ms" if k == 1 else "ms_per_product": round(program_timer(r.stdout) / k * 1e3, 3)}
+    return out
+
+
 def measured_traffic(primitive: str):
     """HBM bytes per call of `primitive` from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
@@ -390,8 +502,12 @@ def main():
     }
     if not a.no_irregular and not a.only:
         out["irregular_f64"] = irregular(a.irregular_log2n, max(3, a.steps))
+    if not a.no_programs and not a.only:
+        out["programs"] = programs(3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        if not a.no_programs and not a.only:
+            out["cpu_baseline"]["programs"] = cpu_programs()
     print(json.dumps(out), flush=True)
 
 
