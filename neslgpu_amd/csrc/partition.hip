@@ -63,15 +63,45 @@ struct IndexSource {
     }
 };
 
+// Intermediate (index, value) pairs between passes.  Four-byte values are
+// interleaved with their index in one 8-byte word, so a bucket's run is one
+// contiguous stream of 8-byte stores (a run of 16 pairs fills a 128-byte
+// line) instead of two half-as-long streams; wider values keep two arrays.
+template <class T, bool PACKED = sizeof(T) == 4>
+struct Pairs {
+    int *idx;
+    T *val;
+    __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
+        q = idx[e];
+        v = val[e];
+    }
+    __device__ __forceinline__ void store(int64_t e, int q, T v) const {
+        idx[e] = q;
+        val[e] = v;
+    }
+};
+template <class T>
+struct Pairs<T, true> {
+    uint2 *p;
+    __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
+        const uint2 w = p[e];
+        q = (int)w.x;
+        v = __builtin_bit_cast(T, w.y);
+    }
+    __device__ __forceinline__ void store(int64_t e, int q, T v) const {
+        p[e] = make_uint2((unsigned)q, __builtin_bit_cast(unsigned, v));
+    }
+};
+
 // One partition pass.  Region r = [r*psize, min((r+1)*psize, n)) of the
 // input is split into children of csize elements (csize divides psize,
 // psize/csize <= 256).  Block b handles tile (b % tiles_per_region) of
 // region (b / tiles_per_region).  Child c's pairs land in
 // [c*csize, c*csize + cnt[c]) of the output.
 template <class T, bool FIRST>
-__global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, const int *__restrict__ in_idx,
-                                                            const T *__restrict__ in_val, int *__restrict__ out_idx,
-                                                            T *__restrict__ out_val, int64_t n, int plog, int clog,
+__global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pairs<T> in,
+                                                            const T *__restrict__ in_val, Pairs<T> out,
+                                                            int64_t n, int plog, int clog,
                                                             int64_t tiles_per_region,
                                                             const unsigned *__restrict__ prev_cnt,
                                                             unsigned *__restrict__ cnt,
@@ -111,8 +141,9 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
                 qq = src.at(e);
                 v[j] = src.value(in_val, e);
             } else {
-                qq = in_idx[e];
-                v[j] = in_val[e];
+                int qi;
+                in.load(e, qi, v[j]);
+                qq = qi;
             }
             const int64_t child = qq >> clog;
             const int64_t d = child - first_child;
@@ -163,8 +194,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, con
         const int64_t cend = (child + 1) * csize < n ? (child + 1) * csize : n;
         const int64_t pos = child * csize + slot;
         if (pos < cend) {
-            out_idx[pos] = s_idx[i];
-            out_val[pos] = s_val[i];
+            out.store(pos, s_idx[i], s_val[i]);
         } else {
             atomicOr(err, 1u);   // more pairs than the block holds: not a permutation
         }
@@ -202,7 +232,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
 // Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
 // first cnt[c] entries of the same range of (idx, val).
 template <class T, bool DIRECT>
-__global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src, const int *__restrict__ in_idx,
+__global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src, Pairs<T> in,
                                                                 const T *__restrict__ in_val, T *__restrict__ d,
                                                                 int64_t n, const unsigned *__restrict__ cnt,
                                                                 unsigned *__restrict__ err) {
@@ -223,8 +253,9 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
             q = src.at(base + i);
             v = src.value(in_val, base + i);
         } else {
-            q = in_idx[base + i];
-            v = in_val[base + i];
+            int qi;
+            in.load(base + i, qi, v);
+            q = qi;
         }
         const int64_t o = q - base;
         if (o >= 0 && o < end - base) {
@@ -258,19 +289,26 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
     int nlog = 0;
     while (((int64_t)1 << nlog) < n) nlog++;
     if (nlog <= kBlkLog) {
-        k_block_scatter<T, true><<<1, kPartThreads, 0, g_stream>>>(src, nullptr, (const T *)s, (T *)d, n, nullptr, err);
+        k_block_scatter<T, true><<<1, kPartThreads, 0, g_stream>>>(src, Pairs<T>{}, (const T *)s, (T *)d, n, nullptr,
+                                                                   err);
         LAUNCHCHECK();
         return;
     }
     if (!scratch) usage_error("permute called without scratch");
     auto up = up256;
     char *b = (char *)scratch;
-    int *idx[2];
-    T *val[2];
-    idx[0] = (int *)b; b += up(4 * n);
-    val[0] = (T *)b; b += up((int64_t)sizeof(T) * n);
-    idx[1] = (int *)b; b += up(4 * n);
-    val[1] = (T *)b; b += up((int64_t)sizeof(T) * n);
+    Pairs<T> buf[2];
+    for (int k = 0; k < 2; k++) {
+        if constexpr (sizeof(T) == 4) {
+            buf[k].p = (uint2 *)b;
+            b += up(8 * n);
+        } else {
+            buf[k].idx = (int *)b;
+            b += up(4 * n);
+            buf[k].val = (T *)b;
+            b += up((int64_t)sizeof(T) * n);
+        }
+    }
     const int levels = (nlog - kBlkLog + 7) / 8;
     int plog = nlog;
     int cur = -1;
@@ -298,17 +336,17 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
             k_col_scan<<<kMaxFan, kColScanThreads, 0, g_stream>>>(hist, grid, cnt);
             LAUNCHCHECK();
             k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, nullptr, (const T *)s, idx[out], val[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+                src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
         } else {
             k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, idx[cur], val[cur], idx[out], val[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
+                src, buf[cur], nullptr, buf[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
         }
         LAUNCHCHECK();
         cur = out;
         plog = clog;
     }
     const int64_t blocks = ((n - 1) >> kBlkLog) + 1;
-    k_block_scatter<T, false><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(src, idx[cur], val[cur], (T *)d, n,
+    k_block_scatter<T, false><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(src, buf[cur], nullptr, (T *)d, n,
                                                                               cnt, err);
     LAUNCHCHECK();
 }
