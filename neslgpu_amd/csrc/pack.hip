@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(const T *__restrict__ s, const 
     __shared__ int s_wcnt[S::WAVES];
     __shared__ int s_wbase[S::WAVES];
 
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;   // no counter: one tile
     __syncthreads();
     const int64_t tile = s_tile;
     const int w = threadIdx.x / kWave;
@@ -141,11 +141,11 @@ void pack(void *d, const void *s, const void *f, int64_t n, int64_t dlim, void *
     if (n <= 0) return;
     if (!scratch) usage_error("pack called without scratch");
     int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    fill_u32(scratch, 0, (kPackHeader + 8 * tiles) / 4);
+    if (tiles > 1) fill_u32(scratch, 0, (kPackHeader + 8 * tiles) / 4);
     int aligned = (((uintptr_t)s | (uintptr_t)f) & 15) == 0;
     k_pack<T, kPackRows><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
         (const T *)s, (const int *)f, (T *)d, n, dlim, (uint64_t *)((char *)scratch + kPackHeader),
-        (unsigned *)scratch, aligned, device_error_word());
+        tiles > 1 ? (unsigned *)scratch : nullptr, aligned, device_error_word());
     LAUNCHCHECK();
 }
 

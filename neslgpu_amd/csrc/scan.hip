@@ -120,7 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(const typename Op::T *__restric
     __shared__ int s_wflag[S::WAVES];
     __shared__ T s_win[S::WAVES];
 
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    // tile ids in launch order (look-back waits only on started tiles); a
+    // single-tile launch passes no counter
+    if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
     __syncthreads();
     const int64_t tile = s_tile;
     const int w = threadIdx.x / kWave;
@@ -311,8 +313,12 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restr
         T b = s_w[0];
         for (int i = 1; i < S::WAVES; i++) b = op(b, s_w[i]);
         __hip_atomic_store(partials + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = prev == gridDim.x - 1;
+        if (counter) {
+            unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = prev == gridDim.x - 1;
+        } else {
+            s_last = true;                  // a one-block launch finishes alone
+        }
     }
     __syncthreads();
     if (s_last && w == 0) {
@@ -359,8 +365,9 @@ static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *s
     if (!scratch) usage_error("scan of %lld elements called without scratch", (long long)n);
     int64_t tiles = (n + S::TILE - 1) / S::TILE;
     if (tiles > 0x7fffffff) usage_error("vector too long");
-    fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<typename Op::T>::kWords * 8) / 4);
-    unsigned *counter = (unsigned *)scratch;
+    // one tile needs neither the counter nor cleared status words
+    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<typename Op::T>::kWords * 8) / 4);
+    unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
     int aligned = al16(s) && al16(d);
     k_scan<Op, R, SEG, REDUCE><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
@@ -410,9 +417,9 @@ void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch)
     int64_t per_block = (n + grid - 1) / grid;
     per_block = (per_block + S::WAVES * S::ROW - 1) / (S::WAVES * S::ROW) * (S::WAVES * S::ROW);
     grid = (int)((n + per_block - 1) / per_block);
-    fill_u32(counter, 0, 1);
-    k_reduce<Op, kScanRows><<<grid, kBlock, 0, g_stream>>>((const T *)s, n, per_block, partials, counter,
-                                                           (T *)result_dev, al16(s));
+    if (grid > 1) fill_u32(counter, 0, 1);
+    k_reduce<Op, kScanRows><<<grid, kBlock, 0, g_stream>>>((const T *)s, n, per_block, partials,
+                                                           grid > 1 ? counter : nullptr, (T *)result_dev, al16(s));
     LAUNCHCHECK();
 }
 

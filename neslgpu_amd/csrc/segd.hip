@@ -41,6 +41,90 @@ __global__ void k_close_wcnt(int *wcnt, const uint32_t *heads, int64_t words) {
         wcnt[words] = words > 0 ? wcnt[words - 1] + __popc(heads[words - 1]) : 0;
 }
 
+// ---- small descriptors: the whole construction in one workgroup
+// (m and the bitmap's word count at most kSmallSegd), instead of a dozen
+// launches; descriptors of short vectors are built very often by
+// interpreted programs.
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallPer = 4;
+constexpr int kSmallSegd = kSmallThreads * kSmallPer;
+
+// Exclusive scan over the workgroup of one value per thread (wrapping
+// unsigned +); every thread must call it.  *total receives the sum.
+__device__ unsigned block_exclusive_scan(unsigned x, unsigned *s_wave, unsigned *total) {
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    constexpr int W = kSmallThreads / kWave;
+    const unsigned inc = wave_inclusive_scan(x, AddU());
+    __syncthreads();                                    // s_wave may still be read from a previous call
+    if (lane == kWave - 1) s_wave[w] = inc;
+    __syncthreads();
+    if (tid < kWave) {
+        const unsigned t = tid < W ? s_wave[tid] : 0u;
+        const unsigned ti = wave_inclusive_scan(t, AddU());
+        if (tid < W) s_wave[tid] = ti - t;
+        if (tid == W - 1) *total = ti;
+    }
+    __syncthreads();
+    return s_wave[w] + inc - x;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restrict__ lens_in, int *lens, int *offs,
+                                                               int *nz, int *wcnt, uint32_t *heads, int64_t n,
+                                                               int64_t m, int64_t words) {
+    __shared__ unsigned s_wave[kSmallThreads / kWave];
+    __shared__ unsigned s_total;
+    __shared__ uint32_t s_heads[kSmallSegd + 8];
+    const int tid = threadIdx.x;
+    unsigned len[kSmallPer], off[kSmallPer], pos[kSmallPer];
+    unsigned lsum = 0, psum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t g = (int64_t)tid * kSmallPer + k;
+        len[k] = g < m ? (unsigned)lens_in[g] : 0u;
+        off[k] = lsum;
+        pos[k] = psum;
+        lsum += len[k];
+        psum += (g < m && (int)len[k] > 0) ? 1u : 0u;
+    }
+    const unsigned lbase = block_exclusive_scan(lsum, s_wave, &s_total);
+    const unsigned ltot = s_total;
+    const unsigned pbase = block_exclusive_scan(psum, s_wave, &s_total);
+    for (int64_t i = tid; i < words + 8; i += kSmallThreads) s_heads[i] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t g = (int64_t)tid * kSmallPer + k;
+        if (g < m) {
+            const int o = (int)(lbase + off[k]);
+            if (lens != lens_in) lens[g] = (int)len[k];
+            offs[g] = o;
+            if ((int)len[k] > 0) {
+                nz[pbase + pos[k]] = (int)g;
+                if (o >= 0 && (int64_t)o < n) atomicOr(&s_heads[o >> 5], 1u << (o & 31));
+            }
+        }
+    }
+    if (tid == 0) offs[m] = (int)ltot;
+    __syncthreads();
+    unsigned hc[kSmallPer], csum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t wd = (int64_t)tid * kSmallPer + k;
+        const uint32_t h = wd < words ? s_heads[wd] : 0u;
+        hc[k] = csum;
+        csum += (unsigned)__popc(h);
+        if (wd < words) heads[wd] = h;
+    }
+    if (tid < 8) heads[words + tid] = 0u;
+    const unsigned cbase = block_exclusive_scan(csum, s_wave, &s_total);
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t wd = (int64_t)tid * kSmallPer + k;
+        if (wd < words) wcnt[wd] = (int)(cbase + hc[k]);
+    }
+    if (tid == 0) wcnt[words] = (int)s_total;
+}
+
 static int grid_cap(int64_t n, int cap = 8192) {
     int g = grid_for(n, kBlock);
     return g > cap ? cap : g;
@@ -63,6 +147,12 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     if (!scratch) usage_error("mke_fov called without scratch");
     int *pos = (int *)((char *)scratch + (scan_scratch_bytes<AddZ>(m > L.words ? m : L.words) + 255) / 256 * 256);
 
+    if (m <= kSmallSegd && L.words <= kSmallSegd) {
+        k_segd_small<<<1, kSmallThreads, 0, g_stream>>>((const int *)lens_in, lens, offs, nz, wcnt, heads, n, m,
+                                                        L.words);
+        LAUNCHCHECK();
+        return;
+    }
     if ((const void *)lens != lens_in && m > 0)
         HIPCHECK(hipMemcpyAsync(lens, lens_in, 4 * (size_t)m, hipMemcpyDeviceToDevice, g_stream));
     scan_unseg<AddZ>(offs, lens, m, scratch);
