@@ -33,6 +33,12 @@ void cvlhip_synchronize(void);
  * resident blocks as the occupancy allows; -1 = one block per tile. */
 void cvlhip_set_blocks_per_cu(int cap);
 
+/* ---- segment descriptors ------------------------------------------------
+ * Byte offset of the m int32 segment lengths inside a descriptor built by
+ * mke_fov (0: the descriptor starts with them, as the serial CVL's does);
+ * a host may read them there instead of copying them out with len_fos. */
+int cvlhip_segd_lengths_offset(void);
+
 /* ---- HIP graphs ------------------------------------------------------
  * A host can record a sequence of CVL calls that return no host data
  * (vector operations, not reductions to a scalar, ext_vu*, v2c, ...) into

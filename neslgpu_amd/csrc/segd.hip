@@ -2,44 +2,18 @@
 // cpy_wus (ref nesl/cvl/serial/facilt.c:106-144, elwise.c:291-298).
 //
 // mke_fov turns the m lengths into the descriptor laid out in segd.hpp:
-// offsets (a device scan of the lengths), the list of non-empty segments,
-// the head bitmap and the per-word head counts.  It runs once per
-// descriptor; every segmented kernel then finds boundaries from the bitmap.
+// offsets (a scan of the lengths), the list of non-empty segments, the head
+// bitmap and the per-word head counts.  It runs once per descriptor; every
+// segmented kernel then finds boundaries from the bitmap.  Small
+// descriptors are built by one workgroup, larger ones by two single-pass
+// look-back kernels.
 #include "cvl_hip.hpp"
 #include "segd.hpp"
 #include "scan.hpp"
+#include "lookback.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
-
-// offs[m] = offs[m-1] + lens[m-1]
-__global__ void k_close_offsets(int *offs, const int *lens, int64_t m) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) offs[m] = m > 0 ? (int)((unsigned)offs[m - 1] + (unsigned)lens[m - 1]) : 0;
-}
-
-// For every non-empty segment g: nz[pos[g]] = g and set its head bit.
-// pos = exclusive count of non-empty segments before g.
-__global__ void k_nz_heads(const int *__restrict__ lens, const int *__restrict__ offs, const int *__restrict__ pos,
-                           int *__restrict__ nz, uint32_t *__restrict__ heads, int64_t m, int64_t n) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; g < m; g += (int64_t)gridDim.x * blockDim.x) {
-        if (lens[g] > 0) {
-            nz[pos[g]] = (int)g;
-            int64_t o = offs[g];
-            if (o >= 0 && o < n) atomicOr(heads + (o >> 5), 1u << (o & 31));
-        }
-    }
-}
-
-__global__ void k_popcount(const uint32_t *__restrict__ heads, int *__restrict__ cnt, int64_t words) {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; w < words; w += (int64_t)gridDim.x * blockDim.x) cnt[w] = __popc(heads[w]);
-}
-
-__global__ void k_close_wcnt(int *wcnt, const uint32_t *heads, int64_t words) {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        wcnt[words] = words > 0 ? wcnt[words - 1] + __popc(heads[words - 1]) : 0;
-}
 
 // ---- small descriptors: the whole construction in one workgroup
 // (m and the bitmap's word count at most kSmallSegd), instead of a dozen
@@ -125,6 +99,134 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restr
     if (tid == 0) wcnt[words] = (int)s_total;
 }
 
+// ---- larger descriptors: two single-pass look-back kernels
+// Pass 1 scans the lengths for (offset, number of non-empty segments
+// before) as one packed pair, writes offs, the lengths copy, nz and the head
+// bits; pass 2 scans the popcounts of the head words into wcnt.  With the
+// clears, four launches instead of a dozen.
+struct AddUs : AddU {
+    __device__ static unsigned sub(unsigned a, unsigned b) { return a - b; }
+};
+
+struct SegPair {                    // (sum of lengths mod 2^32) | (count of non-empty) << 32
+    using T = uint64_t;
+    __host__ __device__ static T neutral() { return 0; }
+    __host__ __device__ static T cvl_id() { return 0; }
+    __device__ T operator()(T a, T b) const {
+        return (uint64_t)(uint32_t)((uint32_t)a + (uint32_t)b) | (((a >> 32) + (b >> 32)) << 32);
+    }
+    __device__ static T sub(T a, T b) {
+        return (uint64_t)(uint32_t)((uint32_t)a - (uint32_t)b) | (((a >> 32) - (b >> 32)) << 32);
+    }
+};
+
+template <class Op>
+__device__ typename Op::T block_tile_prefix(typename Op::T x, typename Op::T *s_wave, typename Op::T *s_tilep,
+                                            uint64_t *status, int64_t tile, unsigned *err,
+                                            typename Op::T *excl_in_block) {
+    // returns the exclusive prefix of this thread's value over all earlier
+    // tiles and threads; *s_tilep receives the tile's inclusive total
+    using T = typename Op::T;
+    using St = TileStatus<T>;
+    const Op op;
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    constexpr int W = kSmallThreads / kWave;
+    const T inc = wave_inclusive_scan(x, op);
+    if (lane == kWave - 1) s_wave[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        T tv = Op::neutral();
+        T wp[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) { wp[i] = tv; tv = op(tv, s_wave[i]); }
+        T P = Op::neutral();
+        if (tile == 0) {
+            if (lane == 0) St::publish(status, 0, kPrefix, tv);
+        } else {
+            if (lane == 0) St::publish(status, tile, kAggregate, tv);
+            P = lookback(status, tile, op, err);
+            if (lane == 0) St::publish(status, tile, kPrefix, op(P, tv));
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < W; i++) s_wave[i] = op(P, wp[i]);
+            *s_tilep = op(P, tv);
+        }
+    }
+    __syncthreads();
+    *excl_in_block = Op::sub(inc, x);
+    return op(s_wave[w], *excl_in_block);
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__restrict__ lens_in, int *lens, int *offs,
+                                                                 int *nz, uint32_t *heads, int64_t n, int64_t m,
+                                                                 uint64_t *status, unsigned *counter,
+                                                                 unsigned *err) {
+    __shared__ unsigned s_tile;
+    __shared__ uint64_t s_wave[kSmallThreads / kWave];
+    __shared__ uint64_t s_total;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t g0 = tile * kSmallSegd + (int64_t)tid * kSmallPer;
+    unsigned len[kSmallPer];
+    uint64_t loc[kSmallPer], sum = 0;
+    const SegPair op;
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t g = g0 + k;
+        len[k] = g < m ? (unsigned)lens_in[g] : 0u;
+        loc[k] = sum;
+        sum = op(sum, (uint64_t)len[k] | ((uint64_t)(g < m && (int)len[k] > 0) << 32));
+    }
+    uint64_t excl;
+    const uint64_t P = block_tile_prefix<SegPair>(sum, s_wave, &s_total, status, tile, err, &excl);
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t g = g0 + k;
+        if (g < m) {
+            const uint64_t q = op(P, loc[k]);
+            const int o = (int)(uint32_t)q;
+            if (lens != lens_in) lens[g] = (int)len[k];
+            offs[g] = o;
+            if ((int)len[k] > 0) {
+                nz[q >> 32] = (int)g;
+                if (o >= 0 && (int64_t)o < n) atomicOr(heads + (o >> 5), 1u << (o & 31));
+            }
+        }
+    }
+    if (tile == (int64_t)gridDim.x - 1 && tid == 0) offs[m] = (int)(uint32_t)s_total;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__restrict__ heads, int *wcnt,
+                                                              int64_t words, uint64_t *status, unsigned *counter,
+                                                              unsigned *err) {
+    __shared__ unsigned s_tile;
+    __shared__ unsigned s_wave[kSmallThreads / kWave];
+    __shared__ unsigned s_total;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t w0 = tile * kSmallSegd + (int64_t)tid * kSmallPer;
+    unsigned loc[kSmallPer], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t wd = w0 + k;
+        loc[k] = sum;
+        sum += wd < words ? (unsigned)__popc(heads[wd]) : 0u;
+    }
+    unsigned excl;
+    const unsigned P = block_tile_prefix<AddUs>(sum, s_wave, &s_total, status, tile, err, &excl);
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int64_t wd = w0 + k;
+        if (wd < words) wcnt[wd] = (int)(P + loc[k]);
+    }
+    if (tile == (int64_t)gridDim.x - 1 && tid == 0) wcnt[words] = (int)s_total;
+}
+
 static int grid_cap(int64_t n, int cap = 8192) {
     int g = grid_for(n, kBlock);
     return g > cap ? cap : g;
@@ -132,8 +234,8 @@ static int grid_cap(int64_t n, int cap = 8192) {
 
 int64_t segd_build_scratch_bytes(int64_t n, int64_t m) {
     SegdLayout L(n, m);
-    int64_t a = scan_scratch_bytes<AddZ>(m > L.words ? m : L.words);
-    return a + 4 * (m + 64);
+    const int64_t ta = (m + kSmallSegd - 1) / kSmallSegd, tb = (L.words + kSmallSegd - 1) / kSmallSegd;
+    return 256 + 16 * (ta + 1) + 16 * (tb + 1) + 256;
 }
 
 void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scratch) {
@@ -145,7 +247,6 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     int *wcnt = (int *)(b + L.wcnt);
     uint32_t *heads = (uint32_t *)(b + L.heads);
     if (!scratch) usage_error("mke_fov called without scratch");
-    int *pos = (int *)((char *)scratch + (scan_scratch_bytes<AddZ>(m > L.words ? m : L.words) + 255) / 256 * 256);
 
     if (m <= kSmallSegd && L.words <= kSmallSegd) {
         k_segd_small<<<1, kSmallThreads, 0, g_stream>>>((const int *)lens_in, lens, offs, nz, wcnt, heads, n, m,
@@ -153,24 +254,26 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
         LAUNCHCHECK();
         return;
     }
-    if ((const void *)lens != lens_in && m > 0)
-        HIPCHECK(hipMemcpyAsync(lens, lens_in, 4 * (size_t)m, hipMemcpyDeviceToDevice, g_stream));
-    scan_unseg<AddZ>(offs, lens, m, scratch);
-    k_close_offsets<<<1, 64, 0, g_stream>>>(offs, lens, m);
-    LAUNCHCHECK();
-    scan_unseg<CountB>(pos, lens, m, scratch);
+    const int64_t ta = (m + kSmallSegd - 1) / kSmallSegd, tb = (L.words + kSmallSegd - 1) / kSmallSegd;
+    unsigned *counters = (unsigned *)scratch;                          // [0] pass 1, [1] pass 2
+    uint64_t *st_a = (uint64_t *)((char *)scratch + 256);
+    uint64_t *st_b = st_a + 2 * (ta + 1);
+    fill_u32(scratch, 0, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4);
     fill_u32(heads, 0, L.words + 8);
     if (m > 0) {
-        k_nz_heads<<<grid_cap(m), kBlock, 0, g_stream>>>(lens, offs, pos, nz, heads, m, n);
+        k_segd_offsets<<<(unsigned)ta, kSmallThreads, 0, g_stream>>>((const int *)lens_in, lens, offs, nz, heads, n, m,
+                                                                     st_a, counters, device_error_word());
         LAUNCHCHECK();
+    } else {
+        fill_u32(offs, 0, 1);
     }
     if (L.words > 0) {
-        k_popcount<<<grid_cap(L.words), kBlock, 0, g_stream>>>(heads, wcnt, L.words);
+        k_segd_wcnt<<<(unsigned)tb, kSmallThreads, 0, g_stream>>>(heads, wcnt, L.words, st_b, counters + 1,
+                                                                  device_error_word());
         LAUNCHCHECK();
-        scan_unseg<AddZ>(wcnt, wcnt, L.words, scratch);
+    } else {
+        fill_u32(wcnt, 0, 1);
     }
-    k_close_wcnt<<<1, 64, 0, g_stream>>>(wcnt, heads, L.words);
-    LAUNCHCHECK();
 }
 
 }  // namespace cvlhip
@@ -180,6 +283,10 @@ using namespace cvlhip;
 extern "C" {
 
 int siz_fos(int n, int m) { return units_for_bytes(SegdLayout(n, m).bytes); }
+
+// Byte offset of the m segment lengths inside a descriptor (they come
+// first), so a host may read them in place instead of through len_fos.
+int cvlhip_segd_lengths_offset(void) { return (int)SegdLayout(0, 0).lens; }
 
 void mke_fov(vec_p sd, vec_p l, int n, int m, vec_p scratch) { segd_build(sd, l, n, m, scratch); }
 int mke_fov_scratch(int n, int m) { return units_for_bytes(segd_build_scratch_bytes(n, m)); }

@@ -228,6 +228,7 @@ struct Vec {
     vec_p p = nullptr;
     int refs = 1;
     Vec *first = nullptr, *second = nullptr;   // pairs
+    Vec *owner = nullptr;   // a view into another vector's memory (LENGTHS of a descriptor)
 };
 
 // The call signature of one instruction: from the opcode table and the
@@ -281,6 +282,9 @@ class Machine {
         arena_.init(static_cast<char *>(base), (size_t)units);
         if (prog_.uses_fused) load_fused();
         if (opt_.graphs) setup_graphs();
+        // backends whose descriptors begin with the m lengths say so
+        auto lens_off = reinterpret_cast<int (*)()>(be_.sym("cvlhip_segd_lengths_offset"));
+        lengths_view_ = lens_off && lens_off() == 0;
         install_constants();
 
         auto it = prog_.labels.find(prog_.entry);
@@ -451,6 +455,8 @@ class Machine {
         if (v->type == T_Pair) {
             unref(v->first);
             unref(v->second);
+        } else if (v->owner) {
+            unref(v->owner);
         } else {
             arena_.release(v->off, v->units);
         }
@@ -725,7 +731,7 @@ class Machine {
         const int rlen = result_len(s, args);
         Vec *d = nullptr;
         if (s.desc == D_Replace) {
-            if (args[0]->refs == 1) {                // REPLACE into the dying source
+            if (args[0]->refs == 1 && !args[0]->owner) {   // REPLACE into the dying source
                 d = args[0];
                 d->refs++;
             } else {
@@ -736,7 +742,7 @@ class Machine {
             unsigned mask = s.inplace_fn();
             for (int j = 0; mask && j < s.nargs; ++j, mask >>= 1) {
                 Vec *a = args[j];
-                if ((mask & 1) && a->refs == 1 && a->type == rtype && a->len == rlen) {
+                if ((mask & 1) && a->refs == 1 && !a->owner && a->type == rtype && a->len == rlen) {
                     d = a;
                     d->refs++;
                     break;
@@ -842,6 +848,18 @@ class Machine {
                 need_stack(1, "LENGTHS");
                 Vec *sd = top();
                 if (sd->type != T_Segdes) die("line %d: applying LENGTHS to a non-segment descriptor", line_);
+                if (lengths_view_) {
+                    // the descriptor starts with its lengths: a read-only view, no copy
+                    Vec *r = new Vec;
+                    r->type = T_Int;
+                    r->len = sd->len;
+                    r->p = sd->p;
+                    r->owner = sd;
+                    sd->refs++;
+                    pop_top();
+                    push(r);
+                    return;
+                }
                 Vec *r = alloc(T_Int, sd->len);
                 {
                     Scratch s(*this, be_.len_fos_scratch(sd->seg_len, sd->len));
@@ -1341,6 +1359,7 @@ class Machine {
     int nfused_ = -1;
     int line_ = 0;
     unsigned long long executed_ = 0;
+    bool lengths_view_ = false;
     Recorder rec_;
     bool graphs_ = false;
     std::map<uint64_t, GraphEntry> graphs_cache_;
