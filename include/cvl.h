@@ -16,10 +16,11 @@
  *     backend, nesl/cvl/serial/facilt.c:92-93), so doubles stay 8-byte
  *     aligned and keep full double precision.
  *   - cvl_bool is a 4-byte int (nesl/include/cvl.h:60-68).
- *   - A segment descriptor starts with the m segment lengths (as in the
- *     serial backend) and is followed by derived data (offsets, a map from
- *     element blocks to segments, and a head-flag bitmap) built once by
- *     mke_fov; siz_fos(n, m) gives its size.
+ *   - A segment descriptor holds a small header (n, m), the m segment
+ *     lengths (the whole of the serial backend's descriptor) and derived
+ *     data (offsets, a map from element blocks to segments, and a
+ *     head-flag bitmap) built once by mke_fov; siz_fos(n, m) gives its
+ *     size.
  *   - Every operation's X_scratch() gives the device scratch, in CVL units,
  *     that the caller passes as `scratch` (the interpreter guarantees it,
  *     nesl/vcode/check_args.c:264-311).

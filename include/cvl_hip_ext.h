@@ -35,8 +35,8 @@ void cvlhip_set_blocks_per_cu(int cap);
 
 /* ---- segment descriptors ------------------------------------------------
  * Byte offset of the m int32 segment lengths inside a descriptor built by
- * mke_fov (0: the descriptor starts with them, as the serial CVL's does);
- * a host may read them there instead of copying them out with len_fos. */
+ * mke_fov (after a 256-byte header that records n and m); a host may read
+ * them there instead of copying them out with len_fos. */
 int cvlhip_segd_lengths_offset(void);
 
 /* ---- HIP graphs ------------------------------------------------------

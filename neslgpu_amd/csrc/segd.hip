@@ -42,9 +42,9 @@ __device__ unsigned block_exclusive_scan(unsigned x, unsigned *s_wave, unsigned 
     return s_wave[w] + inc - x;
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restrict__ lens_in, int *lens, int *offs,
-                                                               int *nz, int *wcnt, uint32_t *heads, int64_t n,
-                                                               int64_t m, int64_t words) {
+__global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restrict__ lens_in, int64_t *hdr, int *lens,
+                                                               int *offs, int *nz, int *wcnt, uint32_t *heads,
+                                                               int64_t n, int64_t m, int64_t words) {
     __shared__ unsigned s_wave[kSmallThreads / kWave];
     __shared__ unsigned s_total;
     __shared__ uint32_t s_heads[kSmallSegd + 8];
@@ -78,7 +78,11 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restr
             }
         }
     }
-    if (tid == 0) offs[m] = (int)ltot;
+    if (tid == 0) {
+        offs[m] = (int)ltot;
+        hdr[0] = n;
+        hdr[1] = m;
+    }
     __syncthreads();
     unsigned hc[kSmallPer], csum = 0;
 #pragma unroll
@@ -158,10 +162,10 @@ __device__ typename Op::T block_tile_prefix(typename Op::T x, typename Op::T *s_
     return op(s_wave[w], *excl_in_block);
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__restrict__ lens_in, int *lens, int *offs,
-                                                                 int *nz, uint32_t *heads, int64_t n, int64_t m,
-                                                                 uint64_t *status, unsigned *counter,
-                                                                 unsigned *err) {
+__global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__restrict__ lens_in, int64_t *hdr,
+                                                                 int *lens, int *offs, int *nz, uint32_t *heads,
+                                                                 int64_t n, int64_t m, uint64_t *status,
+                                                                 unsigned *counter, unsigned *err) {
     __shared__ unsigned s_tile;
     __shared__ uint64_t s_wave[kSmallThreads / kWave];
     __shared__ uint64_t s_total;
@@ -196,7 +200,11 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
             }
         }
     }
-    if (tile == (int64_t)gridDim.x - 1 && tid == 0) offs[m] = (int)(uint32_t)s_total;
+    if (tile == (int64_t)gridDim.x - 1 && tid == 0) {
+        offs[m] = (int)(uint32_t)s_total;
+        hdr[0] = n;
+        hdr[1] = m;
+    }
 }
 
 __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__restrict__ heads, int *wcnt,
@@ -227,6 +235,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__r
     if (tile == (int64_t)gridDim.x - 1 && tid == 0) wcnt[words] = (int)s_total;
 }
 
+__global__ void k_segd_empty(int64_t *hdr, int *offs, int64_t n) {
+    if (threadIdx.x == 0) {
+        hdr[0] = n;
+        hdr[1] = 0;
+        offs[0] = 0;
+    }
+}
+
 static int grid_cap(int64_t n, int cap = 8192) {
     int g = grid_for(n, kBlock);
     return g > cap ? cap : g;
@@ -241,6 +257,7 @@ int64_t segd_build_scratch_bytes(int64_t n, int64_t m) {
 void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scratch) {
     SegdLayout L(n, m);
     char *b = (char *)sd;
+    int64_t *hdr = (int64_t *)(b + L.hdr);
     int *lens = (int *)(b + L.lens);
     int *offs = (int *)(b + L.offs);
     int *nz = (int *)(b + L.nz);
@@ -249,8 +266,8 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     if (!scratch) usage_error("mke_fov called without scratch");
 
     if (m <= kSmallSegd && L.words <= kSmallSegd) {
-        k_segd_small<<<1, kSmallThreads, 0, g_stream>>>((const int *)lens_in, lens, offs, nz, wcnt, heads, n, m,
-                                                        L.words);
+        k_segd_small<<<1, kSmallThreads, 0, g_stream>>>((const int *)lens_in, hdr, lens, offs, nz, wcnt, heads, n,
+                                                        m, L.words);
         LAUNCHCHECK();
         return;
     }
@@ -261,11 +278,12 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     fill_u32(scratch, 0, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4);
     fill_u32(heads, 0, L.words + 8);
     if (m > 0) {
-        k_segd_offsets<<<(unsigned)ta, kSmallThreads, 0, g_stream>>>((const int *)lens_in, lens, offs, nz, heads, n, m,
-                                                                     st_a, counters, device_error_word());
+        k_segd_offsets<<<(unsigned)ta, kSmallThreads, 0, g_stream>>>((const int *)lens_in, hdr, lens, offs, nz, heads,
+                                                                     n, m, st_a, counters, device_error_word());
         LAUNCHCHECK();
-    } else {
-        fill_u32(offs, 0, 1);
+    } else {                                       // no segments: the header and offs[0] = 0
+        k_segd_empty<<<1, 64, 0, g_stream>>>(hdr, offs, n);
+        LAUNCHCHECK();
     }
     if (L.words > 0) {
         k_segd_wcnt<<<(unsigned)tb, kSmallThreads, 0, g_stream>>>(heads, wcnt, L.words, st_b, counters + 1,
@@ -284,8 +302,8 @@ extern "C" {
 
 int siz_fos(int n, int m) { return units_for_bytes(SegdLayout(n, m).bytes); }
 
-// Byte offset of the m segment lengths inside a descriptor (they come
-// first), so a host may read them in place instead of through len_fos.
+// Byte offset of the m segment lengths inside a descriptor, so a host may
+// read them in place instead of copying them out with len_fos.
 int cvlhip_segd_lengths_offset(void) { return (int)SegdLayout(0, 0).lens; }
 
 void mke_fov(vec_p sd, vec_p l, int n, int m, vec_p scratch) { segd_build(sd, l, n, m, scratch); }
@@ -295,11 +313,15 @@ int mke_fov_scratch(int n, int m) { return units_for_bytes(segd_build_scratch_by
 unsigned int mke_fov_inplace(void) { return INPLACE_NONE; }
 
 void len_fos(vec_p l, vec_p sd, int n, int m, vec_p scratch) {
-    (void)n; (void)scratch;
-    if (l != sd && m > 0) HIPCHECK(hipMemcpyAsync(l, sd, 4 * (size_t)m, hipMemcpyDeviceToDevice, g_stream));
+    (void)scratch;
+    if (m > 0)
+        HIPCHECK(hipMemcpyAsync(l, (const char *)sd + SegdLayout(n, m).lens, 4 * (size_t)m, hipMemcpyDeviceToDevice,
+                                g_stream));
 }
 int len_fos_scratch(int n, int m) { (void)n; (void)m; return 0; }
-unsigned int len_fos_inplace(void) { return INPLACE_1; }
+// The lengths sit after the descriptor's header, so writing them over the
+// descriptor would overlap the source.
+unsigned int len_fos_inplace(void) { return INPLACE_NONE; }
 
 // Copy of a whole descriptor (ref elwise.c:291-298 copies the m lengths,
 // which is the whole serial descriptor).

@@ -1,11 +1,13 @@
 // segd.hpp -- the segment descriptor of the HIP backend.
 //
 // The serial CVL's descriptor is the vector of m segment lengths
-// (ref nesl/cvl/serial/facilt.c:106-123).  Here the descriptor *starts*
-// with those lengths (so len_fos and anything that reads a descriptor as
-// lengths keeps working) and continues with data derived once by mke_fov,
-// which the segmented kernels use to find segment boundaries in O(1):
+// (ref nesl/cvl/serial/facilt.c:106-123).  Here the descriptor holds a
+// small header with its own shape, those lengths, and data derived once by
+// mke_fov, which the segmented kernels use to find segment boundaries in
+// O(1):
 //
+//   hdr              int64 n, int64 m (a kernel handed only the descriptor,
+//                    such as a generated fused operation, reads its shape)
 //   lens[m]          segment lengths (int32)
 //   offs[m+1]        exclusive prefix of lens; offs[m] = n
 //   nz[m]            indices of the non-empty segments, in order
@@ -23,14 +25,15 @@
 namespace cvlhip {
 
 struct SegdLayout {
-    int64_t lens, offs, nz, wcnt, heads, bytes;   // byte offsets and total size
+    int64_t hdr, lens, offs, nz, wcnt, heads, bytes;   // byte offsets and total size
     int64_t words;                                 // bitmap words
 
     __host__ __device__ static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
     __host__ __device__ SegdLayout(int64_t n, int64_t m) {
         words = (n + 31) / 32;
-        lens = 0;
+        hdr = 0;
+        lens = 256;
         offs = up256(lens + 4 * m);
         nz = up256(offs + 4 * (m + 1));
         wcnt = up256(nz + 4 * m);
@@ -49,7 +52,7 @@ struct SegdView {
 
     __host__ __device__ SegdView()
         : lens(nullptr), offs(nullptr), nz(nullptr), wcnt(nullptr), heads(nullptr), n(0), m(0) {}
-    __host__ SegdView(const void *sd, int64_t n_, int64_t m_) : n(n_), m(m_) {
+    __host__ __device__ SegdView(const void *sd, int64_t n_, int64_t m_) : n(n_), m(m_) {
         SegdLayout L(n_, m_);
         const char *b = (const char *)sd;
         lens = (const int *)(b + L.lens);
@@ -57,6 +60,12 @@ struct SegdView {
         nz = (const int *)(b + L.nz);
         wcnt = (const int *)(b + L.wcnt);
         heads = (const uint32_t *)(b + L.heads);
+    }
+
+    // The view of a descriptor whose shape only the descriptor knows.
+    __device__ static SegdView from_header(const void *sd) {
+        const int64_t *h = (const int64_t *)sd;
+        return SegdView(sd, h[0], h[1]);
     }
 
     // Is element k the first element of a (non-empty) segment?

@@ -282,9 +282,9 @@ class Machine {
         arena_.init(static_cast<char *>(base), (size_t)units);
         if (prog_.uses_fused) load_fused();
         if (opt_.graphs) setup_graphs();
-        // backends whose descriptors begin with the m lengths say so
+        // backends whose descriptors hold the m lengths contiguously say where
         auto lens_off = reinterpret_cast<int (*)()>(be_.sym("cvlhip_segd_lengths_offset"));
-        lengths_view_ = lens_off && lens_off() == 0;
+        lengths_offset_ = lens_off ? lens_off() : -1;
         install_constants();
 
         auto it = prog_.labels.find(prog_.entry);
@@ -848,12 +848,12 @@ class Machine {
                 need_stack(1, "LENGTHS");
                 Vec *sd = top();
                 if (sd->type != T_Segdes) die("line %d: applying LENGTHS to a non-segment descriptor", line_);
-                if (lengths_view_) {
-                    // the descriptor starts with its lengths: a read-only view, no copy
+                if (lengths_offset_ >= 0) {
+                    // the descriptor holds its lengths: a read-only view, no copy
                     Vec *r = new Vec;
                     r->type = T_Int;
                     r->len = sd->len;
-                    r->p = sd->p;
+                    r->p = static_cast<char *>(sd->p) + lengths_offset_;
                     r->owner = sd;
                     sd->refs++;
                     pop_top();
@@ -1359,7 +1359,7 @@ class Machine {
     int nfused_ = -1;
     int line_ = 0;
     unsigned long long executed_ = 0;
-    bool lengths_view_ = false;
+    int lengths_offset_ = -1;
     Recorder rec_;
     bool graphs_ = false;
     std::map<uint64_t, GraphEntry> graphs_cache_;
