@@ -158,6 +158,8 @@ unsigned cvlhip_check_device_errors(void) {
 
 void cvlhip_synchronize(void) { HIPCHECK(hipStreamSynchronize(g_stream)); }
 
+void *cvlhip_device_error_word(void) { return (void *)device_error_word(); }
+
 // ---- HIP graphs of CVL call sequences
 void *cvlhip_stream_create(void) {
     hipStream_t s = nullptr;

@@ -15,255 +15,10 @@
 #include "lookback.hpp"
 #include "segd.hpp"
 #include "scan.hpp"
+#include "scan_kernel.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
-
-template <class T> struct Vec16;
-template <> struct Vec16<int> { typedef int type __attribute__((ext_vector_type(4))); };
-template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
-
-template <class Op, int R>
-struct ScanShape {
-    using T = typename Op::T;
-    static constexpr int V = 16 / (int)sizeof(T);   // elements per lane per row
-    static constexpr int ROW = kWave * V;            // elements per wave row
-    static constexpr int WAVES = kBlock / kWave;
-    static constexpr int WAVE_ELEMS = R * ROW;
-    static constexpr int TILE = WAVES * WAVE_ELEMS;
-};
-
-template <class Op> struct ordered { static constexpr bool value = false; };
-template <> struct ordered<MaxD> { static constexpr bool value = true; };
-template <> struct ordered<MinD> { static constexpr bool value = true; };
-
-#ifndef CVLHIP_SCAN_ROWS
-#define CVLHIP_SCAN_ROWS 16
-#endif
-constexpr int kScanRows = CVLHIP_SCAN_ROWS;   // rows per wave in one tile
-
-// Load V elements starting at element `base` (neutral beyond n).
-template <class Op, int V>
-__device__ __forceinline__ void load_lane(const typename Op::T *__restrict__ in, int64_t base, int64_t n,
-                                          bool aligned, typename Op::T (&x)[V]) {
-    using T = typename Op::T;
-    if (aligned && base + V <= n) {
-        typename Vec16<T>::type v = *(const typename Vec16<T>::type *)(in + base);
-#pragma unroll
-        for (int k = 0; k < V; k++) x[k] = load_val<Op>(v[k]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < V; k++) x[k] = base + k < n ? load_val<Op>(in[base + k]) : Op::neutral();
-    }
-}
-
-template <class T, int V>
-__device__ __forceinline__ void store_lane(T *__restrict__ out, int64_t base, int64_t n, bool aligned,
-                                           const T (&y)[V]) {
-    if (aligned && base + V <= n) {
-        typename Vec16<T>::type v;
-#pragma unroll
-        for (int k = 0; k < V; k++) v[k] = y[k];
-        *(typename Vec16<T>::type *)(out + base) = v;
-    } else {
-#pragma unroll
-        for (int k = 0; k < V; k++)
-            if (base + k < n) out[base + k] = y[k];
-    }
-}
-
-// Inclusive wave scan of (flag, value) pairs under the segmented operator
-// (fa, a) . (fb, b) = (fa | fb, fb ? b : a op b).
-template <class Op, class T>
-__device__ __forceinline__ void seg_wave_scan(int &f, T &v, Op op) {
-    const T e = Op::neutral();
-#define CVLHIP_SEGSTEP(CTRL, RM)                          \
-    {                                                     \
-        T pv = dpp_move<CTRL, RM>(e, v);                  \
-        int pf = dpp_move<CTRL, RM>(0, f);                \
-        v = f ? v : op(pv, v);                            \
-        f = f | pf;                                       \
-    }
-    CVLHIP_SEGSTEP(kRowShr1, 0xf)
-    CVLHIP_SEGSTEP(kRowShr2, 0xf)
-    CVLHIP_SEGSTEP(kRowShr4, 0xf)
-    CVLHIP_SEGSTEP(kRowShr8, 0xf)
-    CVLHIP_SEGSTEP(kRowBcast15, 0xa)
-    CVLHIP_SEGSTEP(kRowBcast31, 0xc)
-#undef CVLHIP_SEGSTEP
-}
-
-struct SegBits {
-    const uint32_t *heads;   // null when unsegmented
-    const int *wcnt;
-    const int *nz;
-};
-
-// One kernel for: unsegmented exclusive scan (SEG=false, REDUCE=false),
-// segmented exclusive scan (SEG=true, REDUCE=false) and segmented reduce
-// (SEG=true, REDUCE=true; `out` has one element per segment).
-template <class Op, int R, bool SEG, bool REDUCE>
-__global__ __launch_bounds__(kBlock) void k_scan(const typename Op::T *__restrict__ in,
-                                                 typename Op::T *__restrict__ out, int64_t n, SegBits sb,
-                                                 uint64_t *__restrict__ status, unsigned *__restrict__ counter,
-                                                 int aligned_i, unsigned *__restrict__ err) {
-    using T = typename Op::T;
-    using S = ScanShape<Op, R>;
-    using St = TileStatus<T>;
-    constexpr int V = S::V;
-    constexpr uint32_t VMASK = (1u << V) - 1u;
-    const Op op;
-    const bool aligned = aligned_i != 0;
-
-    __shared__ unsigned s_tile;
-    __shared__ T s_wagg[S::WAVES];
-    __shared__ int s_wflag[S::WAVES];
-    __shared__ T s_win[S::WAVES];
-
-    // tile ids in launch order (look-back waits only on started tiles); a
-    // single-tile launch passes no counter
-    if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
-    __syncthreads();
-    const int64_t tile = s_tile;
-    const int w = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int64_t wbase = tile * S::TILE + (int64_t)w * S::WAVE_ELEMS + (int64_t)lane * V;
-
-    T x[R][V];
-    uint32_t hb[R];
-    uint32_t eb[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int64_t base = wbase + (int64_t)r * S::ROW;
-        load_lane<Op, V>(in, base, n, aligned, x[r]);
-        hb[r] = 0;
-        eb[r] = 0;
-        if constexpr (SEG) {
-            if (base < n) {
-                const int64_t wd = base >> 5;
-                const int sh = (int)(base & 31);
-                const uint64_t two = (uint64_t)sb.heads[wd] | ((uint64_t)sb.heads[wd + 1] << 32);
-                hb[r] = (uint32_t)(two >> sh) & VMASK;
-                if constexpr (REDUCE) {
-                    uint32_t nxt = (uint32_t)(two >> (sh + 1)) & VMASK;   // head at element + 1
-                    if (base + V >= n) {                                  // the last element ends its segment
-                        int64_t last = n - 1 - base;
-                        if (last < V) nxt |= 1u << last;
-                        nxt &= (last + 1 >= 32) ? 0xffffffffu : ((1u << (last + 1)) - 1u);
-                    }
-                    eb[r] = nxt;
-                }
-            }
-        }
-    }
-
-    // Wave-local scan of the wave's R rows.
-    T carry = Op::neutral();
-    int cf = 0;
-    uint64_t needs = 0;     // bit r*V+k: element still needs the incoming prefix
-    static_assert(R * V <= 64, "needs mask holds one bit per element of a lane");
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        T a = Op::neutral();
-        int af = 0;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            if (SEG && ((hb[r] >> k) & 1u)) { a = x[r][k]; af = 1; }
-            else a = op(a, x[r][k]);
-        }
-        if constexpr (SEG) seg_wave_scan(af, a, op);
-        else a = wave_inclusive_scan(a, op);
-        T ea = shfl_up1(a);
-        int ef = SEG ? shfl_up1(af) : 0;
-        if (lane == 0) { ea = Op::neutral(); ef = 0; }
-        const T ra = bcast_last(a);
-        const int rf = SEG ? bcast_last(af) : 0;
-        T p = ef ? ea : op(carry, ea);
-        int pf = cf | ef;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            const bool h = SEG && ((hb[r] >> k) & 1u);
-            T y;
-            if constexpr (!REDUCE) {
-                if (h) { y = Op::neutral(); p = x[r][k]; pf = 1; }
-                else { y = p; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
-            } else {
-                if (h) { p = x[r][k]; pf = 1; }
-                else { p = op(p, x[r][k]); if (!pf) needs |= 1ull << (r * V + k); }
-                y = p;
-            }
-            x[r][k] = y;
-        }
-        carry = rf ? ra : op(carry, ra);
-        cf |= rf;
-    }
-
-    if (lane == 0) { s_wagg[w] = carry; s_wflag[w] = cf; }
-    __syncthreads();
-
-    if (w == 0) {
-        T tv = Op::neutral();
-        int tf = 0;
-        T wpv[S::WAVES];
-        int wpf[S::WAVES];
-#pragma unroll
-        for (int i = 0; i < S::WAVES; i++) {
-            wpv[i] = tv;
-            wpf[i] = tf;
-            const T a = s_wagg[i];
-            const int f = s_wflag[i];
-            tv = f ? a : op(tv, a);
-            tf |= f;
-        }
-        T P = Op::neutral();
-        if (tile == 0) {
-            if (lane == 0) St::publish(status, 0, kPrefix, tv);
-        } else {
-            const bool done = SEG && tf;      // a head inside the tile: later tiles need nothing earlier
-            if (lane == 0) St::publish(status, tile, done ? kPrefix : kAggregate, tv);
-            bool first_head = false;
-            if constexpr (SEG) {
-                const int64_t t0 = tile * S::TILE;
-                first_head = (sb.heads[t0 >> 5] >> (t0 & 31)) & 1u;
-            }
-            if (!first_head) P = lookback(status, tile, op, err);
-            if (!done && lane == 0) St::publish(status, tile, kPrefix, op(P, tv));
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < S::WAVES; i++) s_win[i] = wpf[i] ? wpv[i] : op(P, wpv[i]);
-        }
-    }
-    __syncthreads();
-
-    const T inc = s_win[w];
-    const T cid = Op::cvl_id();
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int64_t base = wbase + (int64_t)r * S::ROW;
-        T y[V];
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            T v = x[r][k];
-            if ((needs >> (r * V + k)) & 1ull) v = op(inc, v);
-            y[k] = op(cid, v);
-        }
-        if constexpr (!REDUCE) {
-            store_lane<T, V>(out, base, n, aligned, y);
-        } else {
-#pragma unroll
-            for (int k = 0; k < V; k++) {
-                if ((eb[r] >> k) & 1u) {
-                    const int64_t e = base + k;
-                    const uint32_t wd = sb.heads[e >> 5];
-                    const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);
-                    const int rk = sb.wcnt[e >> 5] + __popc(wd & mask);
-                    out[sb.nz[rk - 1]] = y[k];
-                }
-            }
-        }
-    }
-}
 
 // Unsegmented reduction.  Block b folds a fixed contiguous range (the
 // partition depends only on n), each wave a contiguous quarter of it, rows
@@ -338,14 +93,6 @@ __global__ void k_set1(T *p, T v) {
     if (threadIdx.x == 0) p[0] = v;
 }
 
-// Writes the identity for every empty segment (segmented reductions).
-template <class T>
-__global__ void k_empty_identity(T *__restrict__ out, const int *__restrict__ lens, int64_t m, T id) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; g < m; g += (int64_t)gridDim.x * blockDim.x)
-        if (lens[g] == 0) out[g] = id;
-}
-
 // ------------------------------------------------------------------ host side
 constexpr int64_t kScanHeaderBytes = 256;   // tile counter + padding
 
@@ -371,7 +118,8 @@ static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *s
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
     int aligned = al16(s) && al16(d);
     k_scan<Op, R, SEG, REDUCE><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
-        (const typename Op::T *)s, (typename Op::T *)d, n, sb, status, counter, aligned, device_error_word());
+        ArrayIn<typename Op::T>{(const typename Op::T *)s}, (typename Op::T *)d, n, sb, status, counter, aligned,
+        device_error_word());
     LAUNCHCHECK();
 }
 

@@ -101,6 +101,73 @@ def _used_args(e, acc):
     return acc
 
 
+_REDUCE_OP = {("+_REDUCE", "INT"): "AddZ", ("+_REDUCE", "FLOAT"): "AddD", ("MAX_REDUCE", "INT"): "MaxZ",
+              ("MAX_REDUCE", "FLOAT"): "MaxD", ("MIN_REDUCE", "INT"): "MinZ", ("MIN_REDUCE", "FLOAT"): "MinD"}
+
+
+def _reduce_op(o) -> list:
+    """A super-op whose root reduces an expression over the segments of a
+    descriptor: the CVL's own segmented-reduce kernel (scan_kernel.hpp,
+    16-row tiles, the same association as *_re*) with an input loader that
+    evaluates the expression, so the per-element values are never stored."""
+    k, (rop, r) = o.number, o.reduce
+    op = _REDUCE_OP[(rop, o.result)]
+    rt = CTYPE[o.result]
+    used = sorted(_used_args(o.body, set()))
+    gsrc = S.gather_sources(o.body)
+    gseg = sorted(set(gsrc.values()))
+    fields = [f"const {CTYPE[o.params[i]]} *s{i};" for i in used]
+    fields += [f"const {CTYPE[o.params[i]]} *s{i};" for i in sorted(gsrc) if i not in used]
+    fields += [f"const void *sd{j};" for j in gseg]
+    L = [f"struct InFused{k} {{"]
+    L += [f"    {f}" for f in fields]
+    L += ["    template <class Op, int V>",
+          f"    __device__ __forceinline__ void load(int64_t base, int64_t n, bool, {rt} (&x)[V]) const {{"]
+    for j in gseg:
+        L.append(f"        const long long n{j} = *(const long long *)sd{j};")
+    L += ["#pragma unroll", "        for (int k = 0; k < V; k++) {", "            const int64_t i = base + k;",
+          "            if (i < n) {"]
+    for i in used:
+        L.append(f"                const {CTYPE[o.params[i]]} a{i} = s{i}[i];")
+    L += [f"                x[k] = ({rt})({_expr(o, o.body)});",
+          "            } else {", "                x[k] = Op::neutral();", "            }", "        }", "    }", "};"]
+    hparams = ["vec_p d"] + [f"vec_p s{i}" for i in range(len(o.params))] + ["int len", "vec_p scratch"]
+    init = [f"(const {CTYPE[o.params[i]]} *)s{i}" for i in used]
+    init += [f"(const {CTYPE[o.params[i]]} *)s{i}" for i in sorted(gsrc) if i not in used]
+    init += [f"(const void *)s{j}" for j in gseg]
+    L += [f'extern "C" void fused{k}({", ".join(hparams)}) {{',
+          f"    using Op = cvlhip::{op};",
+          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kScanRows>;",
+          "    hipStream_t st = (hipStream_t)cvlhip_get_stream();"]
+    for i in range(len(o.params)):
+        if i not in used and i not in gsrc and i not in gseg and i != r:
+            L.append(f"    (void)s{i};")
+    L += [f"    const void *sd = (const void *)s{r};",
+          f"    cvlhip::k_empty_identity_hdr<{rt}><<<256, 256, 0, st>>>(({rt} *)d, sd, Op::cvl_id());",
+          f'    fused_check("fused{k}");',
+          "    if (len <= 0) return;",
+          "    const long long tiles = (len + Sh::TILE - 1) / Sh::TILE;",
+          "    unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;",
+          "    uint64_t *status = (uint64_t *)((char *)scratch + 256);",
+          "    if (tiles > 1) {",
+          f"        const long long words = (256 + tiles * cvlhip::TileStatus<{rt}>::kWords * 8) / 4;",
+          "        fused_fill<<<fused_grid(words), 256, 0, st>>>((unsigned *)scratch, words);",
+          "    }",
+          f"    cvlhip::k_scan<Op, cvlhip::kScanRows, true, true, InFused{k}><<<(unsigned)tiles, cvlhip::kBlock, 0, st>>>(",
+          f"        InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len, cvlhip::SegBits{{nullptr, nullptr, nullptr, sd}},",
+          "        status, counter, 0, (unsigned *)cvlhip_device_error_word());",
+          f'    fused_check("fused{k}");',
+          "}",
+          f'extern "C" int fused{k}_scratch(int n, int m) {{',
+          "    (void)m;",
+          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kScanRows>;",
+          "    const long long tiles = ((long long)n + Sh::TILE - 1) / Sh::TILE;",
+          f"    return (int)((256 + tiles * cvlhip::TileStatus<{rt}>::kWords * 8 + 256 + 7) / 8);",
+          "}",
+          f'extern "C" unsigned fused{k}_inplace(void) {{ return 0u; }}', ""]
+    return L
+
+
 def generate(ops, library_name: str = "fused") -> str:
     """HIP source of a fused-operation library."""
     out = [f"// {library_name}: fused VCODE operations generated by neslgpu_amd.fusion for gfx950.",
@@ -108,14 +175,19 @@ def generate(ops, library_name: str = "fused") -> str:
     out += [f"//   {o.text}" for o in ops]
     out += ['#include <hip/hip_runtime.h>', '#include <stdio.h>', '#include <stdlib.h>',
             '#include "cvl.h"', '#include "cvl_hip_ext.h"', '#include "vcode_fused.h"',
-            '#include "elwise_ops.hpp"', "",
+            '#include "elwise_ops.hpp"', '#include "scan_kernel.hpp"', "",
             "static int fused_grid(long long n) {",
             "    long long g = (n + 1023) / 1024;",
             "    return g > 16384 ? 16384 : (g < 1 ? 1 : (int)g);",
             "}", "",
             "template <class T>",
-            "__device__ __forceinline__ T gather(const T *src, int n, int i) {",
-            "    return (unsigned)i < (unsigned)n ? src[i] : (T)0;",
+            "__device__ __forceinline__ T gather(const T *src, long long n, int i) {",
+            "    return i >= 0 && (long long)i < n ? src[i] : (T)0;",
+            "}", "",
+            "__global__ void fused_fill(unsigned *p, long long count) {",
+            "    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;",
+            "         i += (long long)gridDim.x * blockDim.x)",
+            "        p[i] = 0u;",
             "}", "",
             "static void fused_check(const char *what) {",
             "    hipError_t e = hipGetLastError();",
@@ -125,6 +197,9 @@ def generate(ops, library_name: str = "fused") -> str:
             "    }",
             "}", ""]
     for o in ops:
+        if o.reduce:
+            out += _reduce_op(o)
+            continue
         k = o.number
         used = sorted(_used_args(o.body, set()))
         gsrc = S.gather_sources(o.body)
@@ -137,8 +212,8 @@ def generate(ops, library_name: str = "fused") -> str:
         out.append(f"__global__ void __launch_bounds__(256) k_fused{k}({', '.join(params)}) {{")
         out.append("    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;")
         out.append("    const long long stride = (long long)gridDim.x * blockDim.x;")
-        for j in sorted(set(gsrc.values())):      # length of a one-segment descriptor
-            out.append(f"    const int n{j} = sd{j}[0];")
+        for j in sorted(set(gsrc.values())):      # element count from the descriptor's header
+            out.append(f"    const long long n{j} = *(const long long *)sd{j};")
         out.append("    for (; i < n; i += stride) {")
         for i in used:
             out.append(f"        const {CTYPE[o.params[i]]} a{i} = s{i}[i];")
@@ -195,8 +270,16 @@ def generate(ops, library_name: str = "fused") -> str:
                 bits.append("COMPAT1" if t == "SEGDES" else "AGREE1")
             lens.append("|".join(bits) if bits else "NONE")
         rlen = "COMPAT1" if a0_seg else "AGREE1"
+        scr = "1"
+        if o.reduce:
+            r = o.reduce[1]
+            if r >= 5:
+                raise S.DefinitionError("a reduced descriptor must be one of the first five arguments")
+            lens[r] = "COMPAT1"                        # describes the reduced elements
+            rlen = f"AGREE{r + 1}"                     # one result per segment
+            scr = str(r + 1)                           # scratch sized by (n, m) of the descriptor
         out.append(f'    {{VCODE_FUSED_TOKEN, "fused{k}", {n}, 1, {{{atypes}}}, {{{", ".join(lens) or "NONE"}}}, '
-                   f'{{{VTYPE[o.result]}}}, {{{rlen}}}, {{1}}, Elwise{n}}},')
+                   f'{{{VTYPE[o.result]}}}, {{{rlen}}}, {{{scr}}}, Elwise{n}}},')
     out.append("};")
     out.append("cvl_triple_t cvl_funs[] = {")
     for o in ops:

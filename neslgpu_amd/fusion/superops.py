@@ -13,8 +13,11 @@ The right-hand side is an expression tree over the arguments `$i`:
 the elements segment descriptor $s describes.  Segment-descriptor arguments
 only give the length.
 
-One form goes beyond the reference fusion compiler, which fuses elementwise
-code only: `(GATHER TYPE @ $src e $s)` reads $src at the index e evaluates to
+Two forms go beyond the reference fusion compiler, which fuses elementwise
+code only.  At the root, `(+_REDUCE TYPE @ e $s)` (also MAX_REDUCE,
+MIN_REDUCE) reduces e over each segment of $s: one result per segment, the
+products of a sparse matrix-vector product summed per row without writing
+them.  And `(GATHER TYPE @ $src e $s)` reads $src at the index e evaluates to
 (an unsegmented BPERMUTE folded into the fused op; $s is the one-segment
 descriptor of $src, whose length bounds the index: an index outside
 [0, length) gives 0).  $src need not have the length of the other arguments.
@@ -53,6 +56,7 @@ class SuperOp:
     body: object
     result: str = ""        # result type, filled by typecheck
     text: str = ""
+    reduce: tuple = None    # (operator, segd parameter) when the root is a segmented reduction
 
 
 # result type of each elementwise op, given its operand type
@@ -71,6 +75,8 @@ _OPERAND = {
     "LOG": "FLOAT", "SQRT": "FLOAT", "EXP": "FLOAT", "SIN": "FLOAT", "COS": "FLOAT", "TAN": "FLOAT",
     "ASIN": "FLOAT", "ACOS": "FLOAT", "ATAN": "FLOAT", "SINH": "FLOAT", "COSH": "FLOAT", "TANH": "FLOAT",
 }
+
+REDUCES = ("+_REDUCE", "MAX_REDUCE", "MIN_REDUCE")
 
 _HEAD = re.compile(r"^fused\s+OP(\d+)#(\d+)\s*\((.*?)\)\s*=\s*(.*)$")
 
@@ -126,6 +132,18 @@ def parse_line(line: str) -> SuperOp:
     if pos != len(toks):
         raise DefinitionError("trailing tokens")
     op = SuperOp(int(m.group(1)), int(m.group(2)), params, body, text=line.strip())
+    if isinstance(body, Apply) and body.op in REDUCES:
+        if len(body.args) != 2 or not isinstance(body.args[1], Arg) or params[body.args[1].index] != "SEGDES":
+            raise DefinitionError(f"{body.op} needs an expression and a segment descriptor")
+        op.reduce = (body.op, body.args[1].index)
+        op.body = body.args[0]
+        t = typecheck(op)
+        if body.type and body.type != t:
+            raise DefinitionError(f"{body.op} {body.type} of a {t} expression")
+        if t not in ("INT", "FLOAT"):
+            raise DefinitionError(f"{body.op} of {t} is not supported")
+        op.result = t
+        return op
     op.result = typecheck(op)
     return op
 
@@ -163,6 +181,8 @@ def typecheck(op: SuperOp) -> str:
             return a
         if e.op == "RAND":
             raise DefinitionError("RAND cannot be fused")
+        if e.op in REDUCES:
+            raise DefinitionError(f"{e.op} can only be the root of a fused op")
         ts = [ty(x) for x in e.args]
         want = e.type or _OPERAND.get(e.op)
         if want is None:

@@ -7,10 +7,10 @@ import os
 
 import pytest
 
-from neslgpu_amd.build import FUSED_DIR, SUPEROPS_DIR
+from neslgpu_amd.build import FUSED_DIR, PROGRAMS_DIR, SUPEROPS_DIR
 from neslgpu_amd.fusion import codegen, superops as S
 
-DEFS = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")))
+DEFS = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")) + glob.glob(os.path.join(PROGRAMS_DIR, "*.superops")))
 
 
 def test_parse_and_type():
@@ -22,6 +22,10 @@ def test_parse_and_type():
     assert op.result == "FLOAT"
     op = S.parse_line("fused OP0#1 ($0 : BOOL, $1 : FLOAT, $2 : FLOAT) = (SELECT FLOAT @ $0 $1 $2)")
     assert op.result == "FLOAT"
+    op = S.parse_line("fused OP0#1 ($0 : INT, $1 : FLOAT, $2 : SEGDES) = (GATHER FLOAT @ $1 $0 $2)")
+    assert op.result == "FLOAT" and S.gather_sources(op.body) == {1: 2}
+    op = S.parse_line("fused OP1#1 ($0 : FLOAT, $1 : SEGDES) = (+_REDUCE FLOAT @ (* FLOAT @ $0 $0) $1)")
+    assert op.result == "FLOAT" and op.reduce == ("+_REDUCE", 1)
 
 
 @pytest.mark.parametrize("bad", [
@@ -31,6 +35,8 @@ def test_parse_and_type():
     "fused OP0#1 ($0 : INT, $1 : SEGDES) = (DIST INT @ $0 $1)",      # DIST of a vector
     "fused OP0#1 ($0 : INT) = (RAND @ $0)",                          # not fusable
     "fused OP0#1 ($0 : INT) = (+ INT @ $0",                          # unbalanced
+    "fused OP0#1 ($0 : INT, $1 : SEGDES) = (+ INT @ (+_REDUCE INT @ $0 $1) $0)",   # reduce not at the root
+    "fused OP0#1 ($0 : BOOL, $1 : SEGDES) = (+_REDUCE BOOL @ $0 $1)",              # reduce of bools
 ])
 def test_rejects_bad_definitions(bad):
     with pytest.raises((S.DefinitionError, IndexError)):
@@ -43,7 +49,10 @@ def test_reference_definitions_generate(path):
     src = codegen.generate(ops, "x")
     for o in ops:
         assert f'extern "C" void fused{o.number}(' in src
-        assert f"__global__ void __launch_bounds__(256) k_fused{o.number}(" in src
+        if o.reduce:
+            assert f"struct InFused{o.number} {{" in src       # loader for the shared reduce kernel
+        else:
+            assert f"__global__ void __launch_bounds__(256) k_fused{o.number}(" in src
     assert src.count('{VCODE_FUSED_TOKEN, "fused') == len(ops)
 
 
@@ -72,6 +81,10 @@ def test_built_library_tables(path):
         assert [d.arg_type[i] for i in range(d.arg_num)] == [TYPE[t] for t in o.params]
         assert d.result_type[0] == TYPE[o.result]
         assert d.cvl_desc == len(o.params)                 # ElwiseK
-        assert d.result_len[0] == (0x20 if o.params[0] == "SEGDES" else 0x1)
+        if o.reduce:
+            r = o.reduce[1]
+            assert d.result_len[0] == 1 << r and d.arg_len[r] == 0x20 and d.arg_for_scratch[0] == r + 1
+        else:
+            assert d.result_len[0] == (0x20 if o.params[0] == "SEGDES" else 0x1)
         for k in ("", "_scratch", "_inplace"):
             getattr(lib, f"fused{o.number}{k}")

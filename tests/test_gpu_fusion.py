@@ -1,12 +1,17 @@
 """Generated fused kernels against the oracle, one super-op at a time.
 
-Every super-op of the reference programs' definitions is evaluated on
-random inputs by its fused HIP kernel (called through the fused library's
-C entry point) and by the oracle applying the same expression tree one CVL
-elementwise operation at a time.  Integer, boolean and + - * / results must
-match bit for bit; results that pass through a libm transcendental (EXP,
-LOG, ATAN, ...) within 4 ulp, since device and host libm may round
-differently.
+Every super-op of the reference programs' definitions (and of this
+package's SpMV program) is evaluated on random inputs by its fused HIP
+kernel (called through the fused library's C entry point) and by the oracle
+applying the same expression tree one CVL elementwise operation at a time.
+Integer, boolean and + - * / results must match bit for bit; results that
+pass through a libm transcendental (EXP, LOG, ATAN, ...) within 4 ulp, since
+device and host libm may round differently.
+
+A super-op whose root is a segmented reduction must give exactly the bits
+of the CVL reduction (add_red, max_red, ...) applied to the expression's
+values, since it runs the same kernel with the same tiles; max/min also
+equal the oracle's fold, and + is within the compensated-sum bound.
 """
 import ctypes as C
 import glob
@@ -16,13 +21,14 @@ import numpy as np
 import pytest
 
 from gpu_util import host
-from neslgpu_amd.build import FUSED_DIR, SUPEROPS_DIR
+from gpu_util import fp_add_tol_vec
+from neslgpu_amd.build import FUSED_DIR, PROGRAMS_DIR, SUPEROPS_DIR
 from neslgpu_amd.fusion import superops as S
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-DEFS = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")))
+DEFS = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")) + glob.glob(os.path.join(PROGRAMS_DIR, "*.superops")))
 TY = {"INT": "z", "FLOAT": "d", "BOOL": "b"}
 BIN = {"+": "add", "-": "sub", "*": "mul", "/": "div", "%": "mod", "<": "les", "<=": "leq", ">": "grt",
        ">=": "geq", "=": "eql", "!=": "neq", "LSHIFT": "lsh", "RSHIFT": "rsh", "AND": "and", "OR": "ior",
@@ -43,6 +49,12 @@ def oracle_eval(e, args, n):
         return np.full(n, 1 if e.value == "T" else (0 if e.value == "F" else int(e.value)), np.int32)
     if e.op == "DIST":
         return oracle_eval(e.args[0], args, n)
+    if e.op == "GATHER":
+        src = args[e.args[0].index]
+        idx = oracle_eval(e.args[1], args, n).astype(np.int64)
+        ok = (idx >= 0) & (idx < src.size)
+        zero = src.dtype.type(0)
+        return np.where(ok, src[np.clip(idx, 0, max(src.size - 1, 0))] if src.size else zero, zero)
     if e.op == "SELECT":
         f, a, b = (oracle_eval(x, args, n) for x in e.args)
         return O.sel("d" if a.dtype == np.float64 else "z", f, a, b)
@@ -66,6 +78,30 @@ def random_arg(t, n, rng):
     return rng.uniform(0.25, 4.0, n)
 
 
+def check_reduce(cv, o, vals, lens, got):
+    """A fused segmented reduction against the CVL reduction of the same
+    values (same bits) and against the oracle."""
+    import torch
+    rop = {"+_REDUCE": "add", "MAX_REDUCE": "max", "MIN_REDUCE": "min"}[o.reduce[0]]
+    ty = "d" if o.result == "FLOAT" else "z"
+    n, m = vals.size, lens.size
+    sd = cv.segd(lens)
+    dv = torch.from_numpy(np.ascontiguousarray(vals)).to(cv.device)
+    ref = torch.empty(m, dtype=dv.dtype, device=cv.device)
+    name = f"{rop}_re{ty}"
+    getattr(cv.lib, name)(ref, dv, sd, n, m, cv.scratch(getattr(cv.lib, name + "_scratch")(n, m)))
+    r = host(ref)
+    if ty == "d":
+        assert np.array_equal(got.view(np.int64), r.view(np.int64)), o.text
+    else:
+        assert np.array_equal(got, r), o.text
+    if ty == "d" and rop == "add":
+        ref_s, absum = O.segreduce_add_neumaier(vals, lens)
+        assert np.all(np.abs(got - ref_s) <= fp_add_tol_vec(absum, lens)), o.text
+    else:
+        assert np.array_equal(got, O.segreduce(rop, ty, vals, lens)), o.text
+
+
 @pytest.fixture(scope="module")
 def cv():
     from neslgpu_amd.cvl import Cvl
@@ -82,22 +118,46 @@ def test_superops_match_oracle(cv, path):
     for n in (1, 1000, 100_003):
         for o in ops:
             print("superop", name, o.number, "n", n, flush=True)
-            hargs, dargs = [], []
-            for t in o.params:
-                if t == "SEGDES":                      # only its length matters; pass a real descriptor
+            gsrc = S.gather_sources(o.body)
+            gseg = {v: k for k, v in gsrc.items()}
+            hargs, dargs, rlens = [], [], None
+            for i, t in enumerate(o.params):
+                if i in gsrc:                          # a gathered vector: its own length
+                    a = random_arg(t, 37 + n // 3, rng)
+                elif t == "SEGDES" and i in gseg:      # describes the gathered vector
+                    hargs.append(None)
+                    dargs.append(cv.segd(np.array([hargs[gseg[i]].size], np.int32)))
+                    continue
+                elif t == "SEGDES" and o.reduce and i == o.reduce[1]:
+                    rlens = np.diff(np.concatenate([[0], np.sort(rng.integers(0, n + 1, max(1, n // 5))), [n]]))
+                    rlens = rlens.astype(np.int32)
+                    hargs.append(None)
+                    dargs.append(cv.segd(rlens))
+                    continue
+                elif t == "SEGDES":                    # only its length matters
                     hargs.append(None)
                     dargs.append(cv.segd(np.array([n], np.int32)))
+                    continue
                 else:
                     a = random_arg(t, n, rng)
-                    hargs.append(a)
-                    dargs.append(torch.from_numpy(a).to(cv.device))
+                hargs.append(a)
+                dargs.append(torch.from_numpy(a).to(cv.device))
             want = oracle_eval(o.body, hargs, n)
-            out = torch.empty(n, dtype=torch.float64 if o.result == "FLOAT" else torch.int32, device=cv.device)
+            rows = rlens.size if o.reduce else n
+            out = torch.empty(rows, dtype=torch.float64 if o.result == "FLOAT" else torch.int32, device=cv.device)
             fn = getattr(lib, f"fused{o.number}")
             fn.restype = None
             fn.argtypes = [C.c_void_p] * (1 + len(dargs)) + [C.c_int, C.c_void_p]
-            fn(out.data_ptr(), *[d.data_ptr() for d in dargs], n, None)
+            scr = None
+            if o.reduce:
+                sfn = getattr(lib, f"fused{o.number}_scratch")
+                sfn.restype = C.c_int
+                scr = cv.scratch(sfn(n, rlens.size))
+            fn(out.data_ptr(), *[d.data_ptr() for d in dargs], n, None if scr is None else scr.data_ptr())
             got = host(out)
+            if o.reduce:
+                check_reduce(cv, o, want, rlens, got)
+                continue
             if o.result == "FLOAT" and has_transcendental(o.body):
                 assert np.allclose(got, want, rtol=4 * 2.0 ** -52, atol=0, equal_nan=True), o.text
             elif o.result == "FLOAT":
