@@ -1,39 +1,90 @@
-// colscan.hpp -- exclusive scan of each row of a digits x tiles count
-// matrix (row-major, `rows` rows of `cols` counts), used by the chain-free
-// radix passes: after it, entry [d][t] is the number of elements with digit
-// d in tiles before t.  One block per row; the row total goes to totals[d]
-// when totals is non-null.
+// colscan.hpp -- exclusive scan down the columns of a tiles x 256 count
+// matrix, used by the chain-free radix passes.  Row t holds tile t's count
+// of each of the 256 digits (tile-major: a histogram kernel writes its row as
+// one contiguous 1 KiB store, and the reordering kernel reads it back the
+// same way).  After the scan, entry [t][d] is the number of elements with
+// digit d in tiles before t, and totals[d] (when non-null) the column total.
+//
+// One kernel: workgroup g owns rows [g*R, (g+1)*R) and thread d column d.
+// It sums its R counts, publishes the sum in a status word (flag | value,
+// one 64-bit word per (group, digit)), walks back over earlier groups' words
+// until one holds an inclusive prefix, publishes its own inclusive prefix and
+// writes the R exclusive prefixes.  Every group publishes before it waits
+// and groups start in order, so every walk ends; the spin limit only guards
+// against a corrupted status array.
 #pragma once
 
 #include "cvl_hip.hpp"
+#include "lookback.hpp"
 
 namespace cvlhip {
 
-constexpr int kColScanThreads = 1024;
+constexpr int kTileScanRows = 64;
+constexpr int kTileScanCols = 256;
 
-__global__ __launch_bounds__(kColScanThreads) inline void k_col_scan(unsigned *__restrict__ m, int64_t cols,
-                                                                     unsigned *__restrict__ totals) {
-    __shared__ unsigned s_w[kColScanThreads / kWave];
-    unsigned *row = m + (int64_t)blockIdx.x * cols;
-    const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-    unsigned carry = 0;
-    for (int64_t c0 = 0; c0 < cols; c0 += kColScanThreads) {
-        const int64_t c = c0 + tid;
-        const unsigned v = c < cols ? row[c] : 0u;
-        const unsigned inc = wave_inclusive_scan(v, AddU());
-        if (lane == kWave - 1) s_w[w] = inc;
-        __syncthreads();
-        unsigned before = 0, all = 0;
-        for (int i = 0; i < kColScanThreads / kWave; i++) {
-            const unsigned x = s_w[i];
-            if (i < w) before += x;
-            all += x;
-        }
-        if (c < cols) row[c] = carry + before + inc - v;
-        carry += all;
-        __syncthreads();
+__global__ __launch_bounds__(kTileScanCols) inline void k_tile_scan(unsigned *__restrict__ m, int64_t tiles,
+                                                                    unsigned *__restrict__ totals,
+                                                                    uint64_t *__restrict__ status,
+                                                                    unsigned *__restrict__ err) {
+    constexpr int R = kTileScanRows;
+    const int d = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int64_t r0 = g * R;
+    unsigned v[R];
+    unsigned sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        v[r] = r0 + r < tiles ? m[(r0 + r) * kTileScanCols + d] : 0u;
+        sum += v[r];
     }
-    if (totals && tid == 0) totals[blockIdx.x] = carry;
+    uint64_t *mine = status + g * kTileScanCols + d;
+    unsigned excl = 0;
+    if (g == 0) {
+        st_status(mine, ((uint64_t)kPrefix << 32) | sum);
+    } else {
+        st_status(mine, ((uint64_t)kAggregate << 32) | sum);
+        int64_t t = g - 1;
+        int spins = 0;
+        for (;;) {
+            const uint64_t w = ld_status(status + t * kTileScanCols + d);
+            const uint32_t f = (uint32_t)(w >> 32);
+            if (f == kInvalid) {
+                if (++spins >= kSpinLimit) {
+                    atomicOr(err, 2u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += (uint32_t)w;
+            if (f == kPrefix) break;
+            t--;
+            spins = 0;
+        }
+        st_status(mine, ((uint64_t)kPrefix << 32) | (excl + sum));
+    }
+    unsigned run = excl;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (r0 + r < tiles) m[(r0 + r) * kTileScanCols + d] = run;
+        run += v[r];
+    }
+    if (totals && g == (int64_t)gridDim.x - 1) totals[d] = run;
+}
+
+// Status words a scan of `tiles` rows needs (64-bit words).
+inline int64_t tile_scan_status_words(int64_t tiles) {
+    return (int64_t)kTileScanCols * ((tiles + kTileScanRows - 1) / kTileScanRows + 1);
+}
+
+// Scan m (tiles x 256) in place on the library stream; `status` holds
+// tile_scan_status_words(tiles) words and is cleared here.
+inline void tile_scan(unsigned *m, int64_t tiles, unsigned *totals, uint64_t *status) {
+    if (tiles <= 0) return;
+    const int64_t groups = (tiles + kTileScanRows - 1) / kTileScanRows;
+    fill_u32(status, 0, 2 * kTileScanCols * groups);
+    k_tile_scan<<<(unsigned)groups, kTileScanCols, 0, g_stream>>>(m, tiles, totals, status, device_error_word());
+    LAUNCHCHECK();
 }
 
 }  // namespace cvlhip

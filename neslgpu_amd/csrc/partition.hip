@@ -14,8 +14,8 @@
 //                     claims a slot range per bucket with one atomicAdd,
 //                     stages the pairs bucket by bucket in LDS and writes
 //                     each bucket's run contiguously.  One pass splits a
-//                     region into <= 256 children; two passes take 2^28
-//                     elements to 8192-element blocks.
+//                     region into <= 256 children; two passes (8 bits, then
+//                     7) take 2^28 elements to 8192-element blocks.
 //   block scatter     One workgroup per destination block scatters its
 //                     pairs into LDS and writes the block out in order.
 //
@@ -30,15 +30,35 @@
 
 namespace cvlhip {
 
-constexpr int kBlkLog = 13;
-constexpr int64_t kBlk = 1 << kBlkLog;         // final destination block (LDS-resident)
+// Final destination blocks are LDS-resident in the block scatter: 2^kBlkLog4
+// four-byte or 2^kBlkLog8 eight-byte values per workgroup.
+#ifndef CVLHIP_BLK_LOG4
+#define CVLHIP_BLK_LOG4 13
+#endif
+#ifndef CVLHIP_BLK_LOG8
+#define CVLHIP_BLK_LOG8 13
+#endif
+constexpr int kBlkLog4 = CVLHIP_BLK_LOG4, kBlkLog8 = CVLHIP_BLK_LOG8;
+constexpr int kBlkLogMin = kBlkLog4 < kBlkLog8 ? kBlkLog4 : kBlkLog8;
+template <class T>
+constexpr int blk_log() { return sizeof(T) == 4 ? kBlkLog4 : kBlkLog8; }
 #ifndef CVLHIP_PART_ITEMS
 #define CVLHIP_PART_ITEMS 8
 #endif
-constexpr int kPartThreads = 512;
+#ifndef CVLHIP_PART_THREADS
+#define CVLHIP_PART_THREADS 1024
+#endif
+constexpr int kPartThreads = CVLHIP_PART_THREADS;
 constexpr int kPartItems = CVLHIP_PART_ITEMS;
 constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
+// Item j of a thread is element lo + ((j / kPartV) * kPartThreads + tid) *
+// kPartV + j % kPartV of its tile: each lane owns runs of kPartV consecutive
+// elements, read with 16-byte loads where the input is aligned.  (Order
+// within a bucket is free: the indices of a permutation are distinct.)
+constexpr int kPartV = 4;
+static_assert(kPartItems % kPartV == 0, "items per thread come in runs of kPartV");
 constexpr int kMaxFan = 256;                   // children per region per pass
+static_assert(kMaxFan == kTileScanCols, "first-pass histogram rows are scanned 256 wide");
 
 // Where the destination index and the value of input element k come from
 // on the first pass: the index is p[k] (plain permute) or offs[seg(k)] + p[k]
@@ -50,6 +70,7 @@ struct IndexSource {
     bool segmented;
     int vmode;        // 0: s[k]; 1: k; 2: k - offs[seg_of(k)] in vseg
     SegdView vseg;
+    bool vec;         // unsegmented, vmode 0 or 1, p (and s) 16-byte aligned: runs load as vectors
     __device__ __forceinline__ int64_t at(int64_t k) const {
         int64_t q = p[k];
         if (segmented) q += seg.offs[seg.seg_of(k)];
@@ -67,6 +88,34 @@ struct IndexSource {
 // interleaved with their index in one 8-byte word, so a bucket's run is one
 // contiguous stream of 8-byte stores (a run of 16 pairs fills a 128-byte
 // line) instead of two half-as-long streams; wider values keep two arrays.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// One vector element reinterpreted as T.  The element is passed by value,
+// so the cast sees a scalar: __builtin_bit_cast applied directly to an
+// ext_vector subscript (w[c]) reads element 0 whatever c is with this
+// compiler.
+template <class T, class U>
+__device__ __forceinline__ T elem_as(U u) {
+    return __builtin_bit_cast(T, u);
+}
+
+// Four consecutive values from a 16-byte-aligned address.
+template <class T>
+__device__ __forceinline__ void load4(const T *__restrict__ p, T (&x)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        const u32x4 w = *(const u32x4 *)p;
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = elem_as<T>(w[c]);
+    } else {
+        const u64x2 a = *(const u64x2 *)p, b = *(const u64x2 *)(p + 2);
+        x[0] = elem_as<T>(a[0]);
+        x[1] = elem_as<T>(a[1]);
+        x[2] = elem_as<T>(b[0]);
+        x[3] = elem_as<T>(b[1]);
+    }
+}
+
 template <class T, bool PACKED = sizeof(T) == 4>
 struct Pairs {
     int *idx;
@@ -74,6 +123,11 @@ struct Pairs {
     __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
         q = idx[e];
         v = val[e];
+    }
+    // pairs e0 .. e0+3, e0 a multiple of 4 (the buffers are 256-byte aligned)
+    __device__ __forceinline__ void load4(int64_t e0, int (&q)[4], T (&v)[4]) const {
+        cvlhip::load4(idx + e0, q);
+        cvlhip::load4(val + e0, v);
     }
     __device__ __forceinline__ void store(int64_t e, int q, T v) const {
         idx[e] = q;
@@ -87,6 +141,13 @@ struct Pairs<T, true> {
         const uint2 w = p[e];
         q = (int)w.x;
         v = __builtin_bit_cast(T, w.y);
+    }
+    __device__ __forceinline__ void load4(int64_t e0, int (&q)[4], T (&v)[4]) const {
+        const u32x4 a = *(const u32x4 *)(p + e0), b = *(const u32x4 *)(p + e0 + 2);
+        q[0] = (int)a[0]; v[0] = elem_as<T>(a[1]);
+        q[1] = (int)a[2]; v[1] = elem_as<T>(a[3]);
+        q[2] = (int)b[0]; v[2] = elem_as<T>(b[1]);
+        q[3] = (int)b[2]; v[3] = elem_as<T>(b[3]);
     }
     __device__ __forceinline__ void store(int64_t e, int q, T v) const {
         p[e] = make_uint2((unsigned)q, __builtin_bit_cast(unsigned, v));
@@ -131,20 +192,56 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     T v[kPartItems];
     unsigned rk[kPartItems];
     int dg[kPartItems];
+    int64_t qv[kPartItems];
+    bool has[kPartItems];
+#pragma unroll
+    for (int g = 0; g < kPartItems / kPartV; g++) {
+        const int64_t e0 = lo + ((int64_t)g * kPartThreads + tid) * kPartV;
+        if (e0 + kPartV <= rhi && (!FIRST || src.vec)) {
+            int qi[kPartV];
+            T vi[kPartV];
+            if constexpr (FIRST) {
+                load4(src.p + e0, qi);
+                if (src.vmode == 0) {
+                    load4(in_val + e0, vi);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < kPartV; c++) vi[c] = (T)(e0 + c);
+                }
+            } else {
+                in.load4(e0, qi, vi);
+            }
+#pragma unroll
+            for (int c = 0; c < kPartV; c++) {
+                qv[g * kPartV + c] = qi[c];
+                v[g * kPartV + c] = vi[c];
+                has[g * kPartV + c] = true;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kPartV; c++) {
+                const int j = g * kPartV + c;
+                const int64_t e = e0 + c;
+                has[j] = e < rhi;
+                qv[j] = -1;
+                if (has[j]) {
+                    if constexpr (FIRST) {
+                        qv[j] = src.at(e);
+                        v[j] = src.value(in_val, e);
+                    } else {
+                        int qi;
+                        in.load(e, qi, v[j]);
+                        qv[j] = qi;
+                    }
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kPartItems; j++) {
-        const int64_t e = lo + (int64_t)j * kPartThreads + tid;
         dg[j] = -1;
-        if (e < rhi) {
-            int64_t qq;
-            if constexpr (FIRST) {
-                qq = src.at(e);
-                v[j] = src.value(in_val, e);
-            } else {
-                int qi;
-                in.load(e, qi, v[j]);
-                qq = qi;
-            }
+        if (has[j]) {
+            const int64_t qq = qv[j];
             const int64_t child = qq >> clog;
             const int64_t d = child - first_child;
             if (qq >= 0 && qq < n && d >= 0 && d < kMaxFan && (qq >> plog) == region) {
@@ -170,7 +267,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
             const int d = tid * (kMaxFan / kWave) + i;
             s_start[d] = run;
             run += c[i];
-            if (tile_ofs) s_gofs[d] = tile_ofs[(int64_t)d * gridDim.x + bt];
+            if (tile_ofs) s_gofs[d] = tile_ofs[bt * kMaxFan + d];
             else s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
         }
     }
@@ -202,7 +299,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
 }
 
 // Per-tile child counts of the first partition pass (a single region), as
-// a 256 x tiles matrix: the column scan of it gives every tile its slot in
+// a tiles x 256 matrix: the column scan of it gives every tile its slot in
 // every child without atomics, which matter when thousands of tiles share
 // the first pass's few counters.
 __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int64_t n, int clog,
@@ -215,9 +312,17 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
     const int64_t lo = bt * kPartTile;
     int64_t qv[kPartItems];
 #pragma unroll
-    for (int j = 0; j < kPartItems; j++) {
-        const int64_t e = lo + (int64_t)j * kPartThreads + tid;
-        qv[j] = e < n ? src.at(e) : -1;
+    for (int g = 0; g < kPartItems / kPartV; g++) {
+        const int64_t e0 = lo + ((int64_t)g * kPartThreads + tid) * kPartV;
+        if (src.vec && e0 + kPartV <= n) {
+            int qi[kPartV];
+            load4(src.p + e0, qi);
+#pragma unroll
+            for (int c = 0; c < kPartV; c++) qv[g * kPartV + c] = qi[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < kPartV; c++) qv[g * kPartV + c] = e0 + c < n ? src.at(e0 + c) : -1;
+        }
     }
 #pragma unroll
     for (int j = 0; j < kPartItems; j++) {
@@ -226,7 +331,7 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
         if (qq >= 0 && qq < n && d < kMaxFan) atomicAdd(&s_hist[d], 1u);
     }
     __syncthreads();
-    for (int d = tid; d < kMaxFan; d += kPartThreads) hist[(int64_t)d * gridDim.x + bt] = s_hist[d];
+    for (int d = tid; d < kMaxFan; d += kPartThreads) hist[bt * kMaxFan + d] = s_hist[d];
 }
 
 // Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
@@ -236,6 +341,7 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
                                                                 const T *__restrict__ in_val, T *__restrict__ d,
                                                                 int64_t n, const unsigned *__restrict__ cnt,
                                                                 unsigned *__restrict__ err) {
+    constexpr int64_t kBlk = (int64_t)1 << blk_log<T>();
     __shared__ T s_out[kBlk];
     __shared__ unsigned s_mask[kBlk / 32];
     const int tid = threadIdx.x;
@@ -246,23 +352,32 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
     __syncthreads();
     int64_t count = end - base;
     if (!DIRECT && (int64_t)cnt[blk] < count) count = cnt[blk];
-    for (int64_t i = tid; i < count; i += kPartThreads) {
-        int64_t q;
-        T v;
-        if constexpr (DIRECT) {
-            q = src.at(base + i);
-            v = src.value(in_val, base + i);
-        } else {
-            int qi;
-            in.load(base + i, qi, v);
-            q = qi;
-        }
+    auto put = [&](int64_t q, T v) {
         const int64_t o = q - base;
         if (o >= 0 && o < end - base) {
             s_out[o] = v;
             atomicOr(&s_mask[o >> 5], 1u << (o & 31));
         } else {
             atomicOr(err, 1u);
+        }
+    };
+    if constexpr (DIRECT) {
+        for (int64_t i = tid; i < count; i += kPartThreads) put(src.at(base + i), src.value(in_val, base + i));
+    } else {
+        // runs of 4 pairs per lane (base is a multiple of 4)
+        for (int64_t i0 = (int64_t)tid * 4; i0 < count; i0 += (int64_t)kPartThreads * 4) {
+            int qi[4];
+            T vi[4];
+            if (i0 + 4 <= count) {
+                in.load4(base + i0, qi, vi);
+#pragma unroll
+                for (int c = 0; c < 4; c++) put(qi[c], vi[c]);
+            } else {
+                for (int64_t i = i0; i < count; i++) {
+                    in.load(base + i, qi[0], vi[0]);
+                    put(qi[0], vi[0]);
+                }
+            }
         }
     }
     __syncthreads();
@@ -276,16 +391,18 @@ static int64_t up256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 // partition pass (at most 3 passes for n < 2^31).
 int64_t partition_permute_scratch_bytes(int64_t n, int width) {
     const int64_t pair = up256(4 * n) + up256((int64_t)width * n);
-    const int64_t counters = 3 * up256(4 * ((n >> kBlkLog) + 2 + kMaxFan));
-    const int64_t hist = up256(4 * (int64_t)kMaxFan * ((n + kPartTile - 1) / kPartTile + 1));
+    const int64_t counters = 3 * up256(4 * ((n >> kBlkLogMin) + 2 + kMaxFan));
+    const int64_t tiles = (n + kPartTile - 1) / kPartTile + 1;
+    const int64_t hist = up256(4 * (int64_t)kMaxFan * tiles) + up256(8 * tile_scan_status_words(tiles));
     return 2 * pair + counters + hist + 256;
 }
 
 template <class T>
 static void run_partition(void *d, const void *s, IndexSource src, int64_t n, void *scratch) {
     unsigned *err = device_error_word();
-    // Region sizes per pass: from ceil-pow2(n) down to kBlk, at most 8 bits
-    // per pass.
+    constexpr int kBlkLog = blk_log<T>();
+    // Region sizes per pass: from ceil-pow2(n) down to the final block, at
+    // most 8 bits per pass.
     int nlog = 0;
     while (((int64_t)1 << nlog) < n) nlog++;
     if (nlog <= kBlkLog) {
@@ -314,7 +431,10 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
     int cur = -1;
     unsigned *cnt = nullptr, *prev = nullptr;
     for (int l = 0; l < levels; l++) {
-        const int bits = (plog - kBlkLog) - 8 * (levels - 1 - l);   // the first pass takes the odd bits
+        // Full 8-bit passes first, the odd bits last: the first pass takes its
+        // slots from the histogram, later passes claim them with one atomic
+        // per (tile, child), so the narrower fan belongs there.
+        const int bits = l + 1 < levels ? 8 : plog - kBlkLog;
         const int clog = plog - bits;
         const int64_t nchild = ((n - 1) >> clog) + 1;
         prev = cnt;
@@ -331,10 +451,11 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
             // first pass: one region, many tiles -> slots from a histogram + column scan
             unsigned *hist = (unsigned *)b;
             b += up(4 * (int64_t)kMaxFan * grid);
+            uint64_t *status = (uint64_t *)b;
+            b += up(8 * tile_scan_status_words(grid));
             k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
             LAUNCHCHECK();
-            k_col_scan<<<kMaxFan, kColScanThreads, 0, g_stream>>>(hist, grid, cnt);
-            LAUNCHCHECK();
+            tile_scan(hist, grid, cnt, status);
             k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
                 src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
         } else {
@@ -354,14 +475,17 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
 template <class T>
 void partition_permute(void *d, const void *s, const int *p, const void *sd, int64_t n, int64_t m, void *scratch) {
     if (n <= 0) return;
-    IndexSource src{p, SegdView(sd, n, sd ? m : 0), sd != nullptr && m > 1, 0, SegdView()};
+    const bool segmented = sd != nullptr && m > 1;
+    const bool vec = !segmented && ((uintptr_t)p & 15) == 0 && ((uintptr_t)s & 15) == 0;
+    IndexSource src{p, SegdView(sd, n, sd ? m : 0), segmented, 0, SegdView(), vec};
     run_partition<T>(d, s, src, n, scratch);
 }
 
 void partition_invert(int *rank, const int *order, int64_t n, const void *sd, int64_t m, void *scratch) {
     if (n <= 0) return;
     const bool rel = sd != nullptr && m > 1;
-    IndexSource src{order, SegdView(), false, rel ? 2 : 1, rel ? SegdView(sd, n, m) : SegdView()};
+    const bool vec = !rel && ((uintptr_t)order & 15) == 0;
+    IndexSource src{order, SegdView(), false, rel ? 2 : 1, rel ? SegdView(sd, n, m) : SegdView(), vec};
     run_partition<int>(rank, nullptr, src, n, scratch);
 }
 

@@ -29,6 +29,7 @@
 namespace cvlhip {
 
 constexpr int kRadix = 256;
+static_assert(kRadix == kTileScanCols, "per-tile digit counts are scanned 256 wide");
 #ifndef CVLHIP_RANK_THREADS
 #define CVLHIP_RANK_THREADS 1024
 #endif
@@ -59,18 +60,38 @@ __device__ __forceinline__ uint64_t dbl_key(double s, bool up) {
 // ---------------------------------------------------------------- histogram
 // hist[p][d]: number of elements whose digit in pass p is d, for the key
 // passes (from the keys) -- segment passes are counted by k_seg_hist.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Keys are read 16 bytes per lane when the input is 16-byte aligned
+// (`aligned`); the counts do not depend on which lane sees which key.
 template <bool DBL>
-__global__ __launch_bounds__(kBlock) void k_key_hist(const void *__restrict__ s, int64_t n, bool up,
+__global__ __launch_bounds__(kBlock) void k_key_hist(const void *__restrict__ s, int64_t n, bool up, bool aligned,
                                                      unsigned *__restrict__ hist) {
     constexpr int P = DBL ? 8 : 4;
+    constexpr int V = DBL ? 2 : 4;               // keys per 16 bytes
     __shared__ unsigned sh[P][kRadix];
     for (int i = threadIdx.x; i < P * kRadix; i += kBlock) (&sh[0][0])[i] = 0;
     __syncthreads();
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
-        uint64_t key = DBL ? dbl_key(((const double *)s)[k], up) : int_key(((const int *)s)[k], up);
+    auto count = [&](uint64_t key) {
 #pragma unroll
         for (int p = 0; p < P; p++) atomicAdd(&sh[p][(key >> (8 * p)) & 255], 1u);
+    };
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t nv = aligned ? n / V : 0;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < nv; k += stride) {
+        const u32x4 w = ((const u32x4 *)s)[k];
+        if constexpr (DBL) {
+            count(dbl_key(__builtin_bit_cast(double, ((uint64_t)w.y << 32) | w.x), up));
+            count(dbl_key(__builtin_bit_cast(double, ((uint64_t)w.w << 32) | w.z), up));
+        } else {
+            count(int_key((int)w.x, up));
+            count(int_key((int)w.y, up));
+            count(int_key((int)w.z, up));
+            count(int_key((int)w.w, up));
+        }
     }
+    for (int64_t k = nv * V + (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride)
+        count(DBL ? dbl_key(((const double *)s)[k], up) : int_key(((const int *)s)[k], up));
     __syncthreads();
     for (int i = threadIdx.x; i < P * kRadix; i += kBlock)
         if ((&sh[0][0])[i]) atomicAdd(hist + i, (&sh[0][0])[i]);
@@ -104,11 +125,12 @@ __global__ void k_hist_scan(unsigned *hist, unsigned *starts) {
 // ---------------------------------------------------------------- one pass
 struct PassArgs {
     const void *src;         // first pass: the original values; else keys_in
+    bool vec4;               // 4-byte keys (source 0 or 2) at a 16-byte-aligned src
     const int *idx_in;       // null on the first pass (index = position)
     void *keys_out;          // null when the next passes need no keys
     int *idx_out;            // the element order after this pass
     const unsigned *starts;  // 256 bucket starts of this pass
-    const unsigned *tile_ofs;  // 256 x tiles: elements of each digit in earlier tiles
+    const unsigned *tile_ofs;  // tiles x 256: elements of each digit in earlier tiles
     int64_t n;
     int shift;               // digit = (field >> shift) & 255
     int source;              // 0: int values, 1: double values, 2: keys32, 3: keys64, 4: segment number
@@ -127,7 +149,7 @@ __device__ __forceinline__ KeyT pass_key(const PassArgs &a, int64_t e, int ix) {
     return (KeyT)(uint32_t)a.seg.seg_of(ix);
 }
 
-// Per-tile digit counts of one pass, as a 256 x tiles matrix (tile t =
+// Per-tile digit counts of one pass, as a tiles x 256 matrix (tile t =
 // elements [t*8192, (t+1)*8192) of the current order).
 template <class KeyT>
 __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned *__restrict__ hist) {
@@ -137,20 +159,38 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
     __syncthreads();
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = tile * kRankTile;
+    // Item j is element t0 + ((j / 4) * kRankThreads + tid) * 4 + j % 4: runs
+    // of four per lane, one 16-byte load each for 4-byte keys.
+    static_assert(kRankItems % 4 == 0, "runs of four keys per lane");
     KeyT kv[kRankItems];
+    bool in[kRankItems];
 #pragma unroll
-    for (int j = 0; j < kRankItems; j++) {
-        const int64_t e = t0 + (int64_t)j * kRankThreads + tid;
-        kv[j] = 0;
-        if (e < a.n) kv[j] = pass_key<KeyT>(a, e, a.source == 4 ? (a.idx_in ? a.idx_in[e] : (int)e) : 0);
+    for (int g = 0; g < kRankItems / 4; g++) {
+        const int64_t e0 = t0 + ((int64_t)g * kRankThreads + tid) * 4;
+        if (a.vec4 && e0 + 4 <= a.n) {
+            const u32x4 w = *(const u32x4 *)((const uint32_t *)a.src + e0);
+            const uint32_t x[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                kv[g * 4 + c] = (KeyT)(a.source == 0 ? int_key((int)x[c], a.up) : x[c]);
+                in[g * 4 + c] = true;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int64_t e = e0 + c;
+                const int j = g * 4 + c;
+                in[j] = e < a.n;
+                kv[j] = 0;
+                if (in[j]) kv[j] = pass_key<KeyT>(a, e, a.source == 4 ? (a.idx_in ? a.idx_in[e] : (int)e) : 0);
+            }
+        }
     }
 #pragma unroll
-    for (int j = 0; j < kRankItems; j++) {
-        const int64_t e = t0 + (int64_t)j * kRankThreads + tid;
-        if (e < a.n) atomicAdd(&s_hist[(kv[j] >> a.shift) & 255], 1u);
-    }
+    for (int j = 0; j < kRankItems; j++)
+        if (in[j]) atomicAdd(&s_hist[(kv[j] >> a.shift) & 255], 1u);
     __syncthreads();
-    for (int d = tid; d < kRadix; d += kRankThreads) hist[(int64_t)d * gridDim.x + tile] = s_hist[d];
+    for (int d = tid; d < kRadix; d += kRankThreads) hist[tile * kRadix + d] = s_hist[d];
 }
 
 template <class KeyT>
@@ -220,7 +260,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
             run += c;
         }
         s_tot[d] = run;
-        const unsigned prefix = a.tile_ofs[(int64_t)d * gridDim.x + tile];
+        const unsigned prefix = a.tile_ofs[tile * kRadix + d];
         s_gbase[d] = a.starts[d] + prefix;
     }
     __syncthreads();
@@ -284,7 +324,7 @@ __global__ void k_rank_identity(int *rank, int64_t n, SegdView seg, int segmente
 
 // ---------------------------------------------------------------- host driver
 struct RankScratch {
-    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, invert, bytes;
+    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, scan_status, invert, bytes;
     RankScratch(int64_t n, bool dbl) {
         auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
         int64_t kw = dbl ? 8 : 4;
@@ -297,7 +337,8 @@ struct RankScratch {
         starts = up(hist + 4 * 12 * kRadix);
         counters = up(starts + 4 * 12 * kRadix);
         status = up(counters + 4 * 16);
-        invert = up(status + 4 * kRadix * (tiles > 0 ? tiles : 1));
+        scan_status = up(status + 4 * kRadix * (tiles > 0 ? tiles : 1));
+        invert = up(scan_status + 8 * tile_scan_status_words(tiles > 0 ? tiles : 1));
         bytes = up(invert + partition_permute_scratch_bytes(n, 4));
     }
 };
@@ -313,6 +354,7 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     unsigned *hist = (unsigned *)(b + L.hist);
     unsigned *starts = (unsigned *)(b + L.starts);
     unsigned *tile_ofs = (unsigned *)(b + L.status);
+    uint64_t *scan_status = (uint64_t *)(b + L.scan_status);
     const bool segmented = sd != nullptr && m > 1;
     SegdView seg(sd, n, segmented ? m : 0);
     const int key_passes = dbl ? 8 : 4;
@@ -328,8 +370,9 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     {
         int g = grid_for(n, kBlock * 16);
         if (g > 2048) g = 2048;
-        if (dbl) k_key_hist<true><<<g, kBlock, 0, g_stream>>>(s, n, up, hist);
-        else k_key_hist<false><<<g, kBlock, 0, g_stream>>>(s, n, up, hist);
+        const bool aligned = ((uintptr_t)s & 15) == 0;
+        if (dbl) k_key_hist<true><<<g, kBlock, 0, g_stream>>>(s, n, up, aligned, hist);
+        else k_key_hist<false><<<g, kBlock, 0, g_stream>>>(s, n, up, aligned, hist);
         LAUNCHCHECK();
         if (seg_passes) {
             int gs = grid_for(m, kBlock);
@@ -382,14 +425,14 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
             a.src = nullptr;
             a.shift = 8 * (p - key_passes);
         }
+        a.vec4 = (a.source == 0 || a.source == 2) && ((uintptr_t)a.src & 15) == 0;
         a.keys_out = i < last_key_run ? keys[out] : nullptr;
         a.idx_out = idxs[out];
         a.err = device_error_word();
         if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         LAUNCHCHECK();
-        k_col_scan<<<kRadix, kColScanThreads, 0, g_stream>>>(tile_ofs, tiles, nullptr);
-        LAUNCHCHECK();
+        tile_scan(tile_ofs, tiles, nullptr, scan_status);
         if (dbl) k_rank_pass<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
         else k_rank_pass<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
         LAUNCHCHECK();
