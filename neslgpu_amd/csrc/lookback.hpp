@@ -56,11 +56,6 @@ struct TileStatus<T, 2> {
 // raises bit 1 of the device error word.
 constexpr int kSpinLimit = 1 << 22;
 
-// Tiles inspected by each lane per look-back round (the window is 64*K).
-#ifndef CVLHIP_LOOKBACK_K
-#define CVLHIP_LOOKBACK_K 1
-#endif
-constexpr int kLookK = CVLHIP_LOOKBACK_K;
 
 // Exclusive prefix of tile `tile`, computed by one whole wave.  Each round
 // inspects the 64*K tiles before `pos`: lane l reads tiles
@@ -78,12 +73,20 @@ template <class Op> struct seq_combine { static constexpr bool value = false; };
 template <> struct seq_combine<AddD> { static constexpr bool value = true; };
 template <> struct seq_combine<MulD> { static constexpr bool value = true; };
 
+// Tiles inspected by each lane per look-back round (the window is 64*K).
+// A sliding look-back does best with one; a fixed window (seq_combine) must
+// hold a tile with a full prefix before it can finish, and the prefix
+// frontier trails the newest tiles by more than 64 of them at full
+// occupancy, so it inspects two per lane.
+template <class Op>
+struct lookback_k { static constexpr int value = seq_combine<Op>::value ? 2 : 1; };
+
 template <class Op>
 __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile, Op op,
                                    unsigned *__restrict__ err) {
     using T = typename Op::T;
     using St = TileStatus<T>;
-    constexpr int K = kLookK;
+    constexpr int K = lookback_k<Op>::value;
     const int lane = lane_id();
     T acc = Op::neutral();
     int64_t pos = tile - 1;
@@ -133,9 +136,21 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
                 continue;
             }
             if (!pm && lane == 0) atomicOr(err, 2u);
+            // Strict left fold from the prefix tile: the later tiles of lane
+            // Ls, then those of lanes Ls-1 .. 0, each lane's earliest first.
             const int Ls = L < kWave ? L : kWave - 1;
-            T w = shfl(c, Ls);                     // the prefix, then later tiles in order
-            for (int l = Ls - 1; l >= 0; l--) w = op(w, shfl(c, l));
+            const int jL = L < kWave ? shfl(fj, Ls) : K - 1;
+            T w = Op::neutral();
+            bool first = true;
+            for (int l = Ls; l >= 0; l--) {
+#pragma unroll
+                for (int j = K - 1; j >= 0; j--) {
+                    if (l == Ls && j > jL) continue;
+                    const T x = op(Op::neutral(), shfl(v[j], l));
+                    w = first ? x : op(w, x);
+                    first = false;
+                }
+            }
             return w;
         }
         T r = shfl(c, kWave - 1 - lane);          // lane 0 now holds the earliest tiles

@@ -54,6 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(const T *__restrict__ s, const 
     __shared__ unsigned s_tile;
     __shared__ int s_wcnt[S::WAVES];
     __shared__ int s_wbase[S::WAVES];
+    __shared__ T s_run[S::WAVES][S::ROW];   // one row's kept values, compacted
 
     if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;   // no counter: one tile
     __syncthreads();
@@ -84,12 +85,14 @@ __global__ __launch_bounds__(kBlock) void k_pack(const T *__restrict__ s, const 
         }
     }
     int carry = 0;
+    int rtot[R];         // kept values per row of the wave
 #pragma unroll
     for (int r = 0; r < R; r++) {
         int c = __popc(fb[r]);
         int inc = wave_inclusive_scan(c, add);
         pos[r] = carry + inc - c;
-        carry += bcast_last(inc);
+        rtot[r] = bcast_last(inc);
+        carry += rtot[r];
     }
     if (lane == 0) s_wcnt[w] = carry;
     __syncthreads();
@@ -111,18 +114,32 @@ __global__ __launch_bounds__(kBlock) void k_pack(const T *__restrict__ s, const 
         }
     }
     __syncthreads();
+    // Row by row, the wave compacts its kept values in LDS and writes them
+    // as one contiguous run: every store instruction covers 64 consecutive
+    // output elements.
     const int64_t ob = s_wbase[w];
+    T *run = s_run[w];
+    int rbase = 0;            // kept values of earlier rows of this wave
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        int64_t o = ob + pos[r];
+        const int rcount = rtot[r];
+        int o = pos[r] - rbase;
 #pragma unroll
         for (int k = 0; k < V; k++) {
-            if ((fb[r] >> k) & 1u) {
-                if (o < dlim) d[o] = x[r][k];
-                else *err = 1u;
-                o++;
-            }
+            if ((fb[r] >> k) & 1u) run[o++] = x[r][k];
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < rcount; i += kWave) {
+            const int64_t at = ob + rbase + i;
+            if (at < dlim) d[at] = run[i];
+            else *err = 1u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        rbase += rcount;
     }
 }
 

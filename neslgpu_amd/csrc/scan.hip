@@ -140,6 +140,72 @@ void scan_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void
     launch_scan<Op, true, false>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
 }
 
+// Unsegmented reduction for operators whose result does not depend on the
+// order of the operands (all but double max/min): wave v of the grid folds
+// rows v, v + W, v + 2W, ... (W = waves in the grid), so at any moment the
+// whole grid reads one contiguous window of memory, as a plain streaming
+// read does.  The block partials are combined in block order, so the
+// result is the same on every run.
+template <class Op, int U>
+__global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T *__restrict__ in, int64_t n,
+                                                           typename Op::T *__restrict__ partials,
+                                                           unsigned *__restrict__ counter,
+                                                           typename Op::T *__restrict__ result, int aligned_i) {
+    using T = typename Op::T;
+    using S = ScanShape<Op, 1>;
+    constexpr int V = S::V;
+    const Op op;
+    const bool aligned = aligned_i != 0;
+    __shared__ T s_w[S::WAVES];
+    __shared__ bool s_last;
+    const int w = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t nw = (int64_t)gridDim.x * S::WAVES;
+    T acc = Op::neutral();
+    for (int64_t r0 = (int64_t)blockIdx.x * S::WAVES + w; r0 * S::ROW < n; r0 += U * nw) {
+        T x[U][V];
+#pragma unroll
+        for (int u = 0; u < U; u++) load_lane<Op, V>(in, (r0 + u * nw) * S::ROW + (int64_t)lane * V, n, aligned, x[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < V; k++) acc = op(acc, x[u][k]);
+    }
+    acc = wave_reduce(acc, op);
+    if (lane == 0) s_w[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T b = s_w[0];
+        for (int i = 1; i < S::WAVES; i++) b = op(b, s_w[i]);
+        __hip_atomic_store(partials + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (counter) {
+            unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = prev == gridDim.x - 1;
+        } else {
+            s_last = true;
+        }
+    }
+    __syncthreads();
+    if (s_last && w == 0) {
+        T tot = Op::neutral();
+        for (int i0 = 0; i0 < (int)gridDim.x; i0 += kWave) {
+            const int i = i0 + lane;
+            T v = i < (int)gridDim.x ? __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : Op::neutral();
+            tot = op(tot, wave_reduce(v, op));
+        }
+        if (lane == 0) result[0] = op(Op::cvl_id(), tot);
+    }
+}
+
+#ifndef CVLHIP_REDUCE_GRID
+#define CVLHIP_REDUCE_GRID 1024
+#endif
+#ifndef CVLHIP_REDUCE_U
+#define CVLHIP_REDUCE_U 8
+#endif
+static_assert(CVLHIP_REDUCE_GRID <= 1024, "the reduction scratch holds 1024 block partials");
+
 template <class Op>
 int64_t reduce_scratch_bytes(int64_t n) {
     (void)n;
@@ -159,6 +225,19 @@ void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch)
         k_set1<T><<<1, 64, 0, g_stream>>>((T *)result_dev, Op::cvl_id());
         LAUNCHCHECK();
         return;
+    }
+    if constexpr (!ordered<Op>::value) {
+#ifndef CVLHIP_REDUCE_CHUNKED
+        using S1 = ScanShape<Op, 1>;
+        constexpr int64_t step = (int64_t)S1::WAVES * S1::ROW * CVLHIP_REDUCE_U;   // elements per block per step
+        const int64_t want = (n + step - 1) / step;
+        const int grid = (int)(want < CVLHIP_REDUCE_GRID ? want : CVLHIP_REDUCE_GRID);
+        if (grid > 1) fill_u32(counter, 0, 1);
+        k_reduce_strided<Op, CVLHIP_REDUCE_U><<<grid, kBlock, 0, g_stream>>>(
+            (const T *)s, n, partials, grid > 1 ? counter : nullptr, (T *)result_dev, al16(s));
+        LAUNCHCHECK();
+        return;
+#endif
     }
     int64_t tiles = (n + S::TILE - 1) / S::TILE;
     int grid = (int)(tiles < 1024 ? tiles : 1024);
