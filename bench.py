@@ -280,7 +280,7 @@ def program_workloads():
         ("spmv", spmv, spmv_in, None, False, 10, "y = A x, CSR 2^22 rows, power-law lengths (mean 16, "
          "max 2^16), 2^26 nnz, float64; BPERMUTE, *, +_REDUCE"),
         ("spmv_fused", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), False, 10,
-         "same, gather and multiply fused in one kernel"),
+         "same, gather, multiply and segmented sum fused in one kernel"),
         ("spmv_fused_graph", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), True, 10,
          "same, fused block replayed as a HIP graph"),
     ], rows
