@@ -171,7 +171,8 @@ def irregular(log2n: int, reps: int) -> dict:
     """Segmented float64 +-scan and max-reduce over Zipf(1.5) segment lengths
     clipped to [1, 2^20] (2^26 values uniform in [-1, 1]): every schedule
     timed (median of `reps`, HIP events), the learned dispatcher's pick, the
-    exhaustive autotuner's best and the CVL entry points add_sed / max_red."""
+    exhaustive autotuner's best, the CVL entry points add_sed / max_red, and
+    both operations in one pass (cvlhip_add_sed_max_red)."""
     import torch
     from neslgpu_amd import dispatch as D
     from neslgpu_amd.cvl import Cvl
@@ -189,7 +190,8 @@ def irregular(log2n: int, reps: int) -> dict:
     disp = None
     if os.path.exists(D.MODEL_PATH):
         disp = D.Dispatcher(cv).load()
-    scr = cv.scratch(max(L.add_sed_scratch(n, m), L.max_red_scratch(n, m), L.cvlhip_seg_scratch(n, m)))
+    scr = cv.scratch(max(L.add_sed_scratch(n, m), L.max_red_scratch(n, m), L.cvlhip_seg_scratch(n, m),
+                         L.cvlhip_add_sed_max_red_scratch(n, m)))
     out = {"workload": f"2^{log2n} float64 uniform [-1,1], Zipf(1.5) segment lengths clipped to [1,2^20]",
            "n": n, "segments": m, "arms": D.ARMS}
     tot = {"learned": 0.0, "autotuned": 0.0, "cvl_entry": 0.0}
@@ -214,6 +216,12 @@ def irregular(log2n: int, reps: int) -> dict:
             rec["learned_ms"] = round(float(t[pick]), 4)
             tot["learned"] += float(t[pick])
         out[f"{op}_{kind}"] = rec
+    # both operations in one pass over the values (cvlhip_add_sed_max_red)
+    pair = lambda: L.cvlhip_add_sed_max_red(d, r, x, sd, n, m, scr)
+    tp = float(np.median([time_gpu(pair, 1, torch) for _ in range(reps)]))
+    tot["one_pass_pair"] = tp
+    out["one_pass_pair"] = {"ms": round(tp, 4), "compulsory_bytes": 16 * n + 4 * m + 8 * m,
+                            "GB/s": round((16 * n + 12 * m) / tp / 1e6, 1)}
     out["total_ms"] = {k: round(v, 4) for k, v in tot.items()}
     out["fixed_arm_total_ms"] = {D.ARMS[a]: round(float(fixed[a]), 4) for a in range(len(D.ARMS))}
     if disp is not None:

@@ -81,6 +81,14 @@ int cvlhip_seg_arms(void);
 void cvlhip_seg_scan(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
 void cvlhip_seg_reduce(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
 int cvlhip_seg_scratch(int n, int m);
+/* add_sed and max_red of the same values in one pass, reading s once:
+ * scan_d gets add_sed(s, sd), red_d gets max_red(s, sd) (n elements, m
+ * segments; scratch: cvlhip_add_sed_max_red_scratch units).  No
+ * counterpart in the reference, whose programs call the two CVL functions
+ * (nesl/cvl/serial/vprims.c:62-109 and 155-201) one after the other; the
+ * scan differs from add_sed only in the association of its float sums. */
+void cvlhip_add_sed_max_red(vec_p scan_d, vec_p red_d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
+int cvlhip_add_sed_max_red_scratch(int n, int m);
 /* Descriptor statistics for the dispatcher (host floats, see segstrat.hip). */
 int cvlhip_seg_features_count(void);
 void cvlhip_segd_features(vec_p sd, int n, int m, float *out);

@@ -184,17 +184,29 @@ __device__ __forceinline__ int dpp_i32(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, CTRL, ROW_MASK, BANK_MASK, false);
 }
 
+// A value seen as its 32-bit words (for moving wider types across lanes).
+template <class T>
+struct Words {
+    static_assert(sizeof(T) % 4 == 0, "cross-lane moves of whole 32-bit words");
+    int w[sizeof(T) / 4];
+};
+
 template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf, class T>
 __device__ __forceinline__ T dpp_move(T old, T src) {
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "dpp on 32/64-bit values");
     if constexpr (sizeof(T) == 4) {
         return __builtin_bit_cast(T, dpp_i32<CTRL, ROW_MASK, BANK_MASK>(
                                            __builtin_bit_cast(int, old), __builtin_bit_cast(int, src)));
-    } else {
+    } else if constexpr (sizeof(T) == 8) {
         uint64_t o = __builtin_bit_cast(uint64_t, old), s = __builtin_bit_cast(uint64_t, src);
         int lo = dpp_i32<CTRL, ROW_MASK, BANK_MASK>((int)(uint32_t)o, (int)(uint32_t)s);
         int hi = dpp_i32<CTRL, ROW_MASK, BANK_MASK>((int)(uint32_t)(o >> 32), (int)(uint32_t)(s >> 32));
         return __builtin_bit_cast(T, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    } else {
+        const Words<T> o = __builtin_bit_cast(Words<T>, old), s = __builtin_bit_cast(Words<T>, src);
+        Words<T> r;
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); i++) r.w[i] = dpp_i32<CTRL, ROW_MASK, BANK_MASK>(o.w[i], s.w[i]);
+        return __builtin_bit_cast(T, r);
     }
 }
 
@@ -220,11 +232,16 @@ template <class T>
 __device__ __forceinline__ T shfl(T v, int src_lane) {
     if constexpr (sizeof(T) == 4) {
         return __builtin_bit_cast(T, __shfl(__builtin_bit_cast(int, v), src_lane, kWave));
-    } else {
+    } else if constexpr (sizeof(T) == 8) {
         uint64_t u = __builtin_bit_cast(uint64_t, v);
         int lo = __shfl((int)(uint32_t)u, src_lane, kWave);
         int hi = __shfl((int)(uint32_t)(u >> 32), src_lane, kWave);
         return __builtin_bit_cast(T, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    } else {
+        Words<T> u = __builtin_bit_cast(Words<T>, v);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); i++) u.w[i] = __shfl(u.w[i], src_lane, kWave);
+        return __builtin_bit_cast(T, u);
     }
 }
 

@@ -31,21 +31,32 @@ struct TileStatus<T, 1> {
     }
 };
 
-template <class T>
-struct TileStatus<T, 2> {
-    static constexpr int kWords = 2;
+// Values of two or more words: every word carries the flag; a reader
+// accepts the value only when all flags agree.
+template <class T, int WORDS>
+struct TileStatus {
+    static_assert(WORDS >= 2 && sizeof(T) == 4 * WORDS, "status of whole 32-bit words");
+    static constexpr int kWords = WORDS;
     __device__ static void publish(uint64_t *st, int64_t t, uint32_t flag, T v) {
-        uint64_t u = __builtin_bit_cast(uint64_t, v);
-        uint64_t f = (uint64_t)flag << 32;
-        st_status(st + 2 * t, f | (uint32_t)u);
-        st_status(st + 2 * t + 1, f | (uint32_t)(u >> 32));
+        const Words<T> u = __builtin_bit_cast(Words<T>, v);
+        const uint64_t f = (uint64_t)flag << 32;
+#pragma unroll
+        for (int i = 0; i < WORDS; i++) st_status(st + WORDS * t + i, f | (uint32_t)u.w[i]);
     }
     __device__ static uint32_t read(const uint64_t *st, int64_t t, T &v) {
-        uint64_t w0 = ld_status(st + 2 * t);
-        uint64_t w1 = ld_status(st + 2 * t + 1);
-        uint32_t f0 = (uint32_t)(w0 >> 32), f1 = (uint32_t)(w1 >> 32);
-        if (f0 != f1) return kInvalid;
-        v = __builtin_bit_cast(T, ((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull));
+        Words<T> u;
+        uint32_t f0 = 0;
+        bool agree = true;
+#pragma unroll
+        for (int i = 0; i < WORDS; i++) {
+            const uint64_t w = ld_status(st + WORDS * t + i);
+            const uint32_t f = (uint32_t)(w >> 32);
+            if (i == 0) f0 = f;
+            agree = agree && f == f0;
+            u.w[i] = (int)(uint32_t)w;
+        }
+        if (!agree) return kInvalid;
+        v = __builtin_bit_cast(T, u);
         return f0;
     }
 };

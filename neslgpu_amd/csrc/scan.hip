@@ -300,6 +300,44 @@ void reduce_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m,
     launch_scan_r<Op, 8, true, true>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
 }
 
+// add_sed and max_red of the same values in one pass (one read of s): the
+// scan runs on component a and the reduction on component b of PairD.
+// Rows per wave of its tiles: a pair is four registers per element, so 8
+// rows keep the occupancy of the 16-row double scan.
+#ifndef CVLHIP_PAIR_ROWS
+#define CVLHIP_PAIR_ROWS 8
+#endif
+constexpr int kPairRows = CVLHIP_PAIR_ROWS;
+using AddMaxD = PairOp<AddD, MaxD>;
+
+int64_t add_max_pair_scratch_bytes(int64_t n) {
+    using S = ScanShape<AddMaxD, kPairRows>;
+    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
+    return kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8 + 256;
+}
+
+void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd, int64_t n, int64_t m,
+                  void *scratch) {
+    using S = ScanShape<AddMaxD, kPairRows>;
+    if (m <= 0) return;
+    if (!scratch) usage_error("add_sed/max_red pair called without scratch");
+    SegdView v(sd, n, m);
+    int g = grid_for(m, kBlock);
+    if (g > 4096) g = 4096;
+    k_empty_identity<double><<<g, kBlock, 0, g_stream>>>(red_d, v.lens, m, MaxD::cvl_id());
+    LAUNCHCHECK();
+    if (n <= 0) return;
+    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
+    if (tiles > 0x7fffffff) usage_error("vector too long");
+    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8) / 4);
+    unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
+    uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
+    k_scan<AddMaxD, kPairRows, true, true, PairIn><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
+        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz}, status, counter, al16(s),
+        device_error_word());
+    LAUNCHCHECK();
+}
+
 #define CVLHIP_INST_SEG(OP)                                                                         \
     template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);      \
     template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);    \
@@ -356,6 +394,14 @@ using namespace cvlhip;
     unsigned int OPN##_re##TY##_inplace(void) { return INPLACE_NONE; }
 
 extern "C" {
+void cvlhip_add_sed_max_red(vec_p scan_d, vec_p red_d, vec_p s, vec_p sd, int n, int m, vec_p scratch) {
+    add_max_pair((double *)scan_d, (double *)red_d, (const double *)s, sd, n, m, scratch);
+}
+int cvlhip_add_sed_max_red_scratch(int n, int m) {
+    (void)m;
+    return units_for_bytes(add_max_pair_scratch_bytes(n));
+}
+
 CVLHIP_SCANS(add, z, AddZ)
 CVLHIP_SCANS(add, d, AddD)
 CVLHIP_SCANS(mul, z, MulZ)

@@ -14,10 +14,13 @@ template <class T> struct Vec16;
 template <> struct Vec16<int> { typedef int type __attribute__((ext_vector_type(4))); };
 template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
 
+// Elements each lane holds per row: one 16-byte load's worth, by default.
+template <class Op> struct lane_values { static constexpr int value = 16 / (int)sizeof(typename Op::T); };
+
 template <class Op, int R>
 struct ScanShape {
     using T = typename Op::T;
-    static constexpr int V = 16 / (int)sizeof(T);   // elements per lane per row
+    static constexpr int V = lane_values<Op>::value;   // elements per lane per row
     static constexpr int ROW = kWave * V;            // elements per wave row
     static constexpr int WAVES = kBlock / kWave;
     static constexpr int WAVE_ELEMS = R * ROW;
@@ -102,9 +105,68 @@ struct ArrayIn {
     }
 };
 
+// A loader that also says where results go when one pass produces two of
+// them (io_dual): the kernel then writes the exclusive scan of component a
+// and the per-segment reduction of component b itself.
+template <class In> struct io_dual { static constexpr bool value = false; };
+
+// Two double results of the same values in one pass: component a runs a
+// segmented scan with A, component b a segmented reduction with B.
+struct PairD {
+    double a, b;
+};
+template <class A, class B>
+struct PairOp {
+    using T = PairD;
+    __host__ __device__ static T neutral() { return {A::neutral(), B::neutral()}; }
+    __host__ __device__ static T cvl_id() { return {A::cvl_id(), B::cvl_id()}; }
+    __host__ __device__ T operator()(T x, T y) const { return {A()(x.a, y.a), B()(x.b, y.b)}; }
+};
+template <class A, class B>
+struct seq_combine<PairOp<A, B>> {
+    static constexpr bool value = seq_combine<A>::value || seq_combine<B>::value;
+};
+// Two pairs per lane and row: one 16-byte load of two doubles.
+#ifndef CVLHIP_PAIR_V
+#define CVLHIP_PAIR_V 2
+#endif
+template <class A, class B>
+struct lane_values<PairOp<A, B>> { static constexpr int value = CVLHIP_PAIR_V; };
+
+// Reads a double vector as pairs (x, x) and stores the scan of a into
+// `scan` and the reduction of b into `red` (one value per segment).
+struct PairIn {
+    const double *p;
+    double *scan;
+    double *red;
+    template <class Op, int V>
+    __device__ __forceinline__ void load(int64_t base, int64_t n, bool aligned, PairD (&x)[V]) const {
+        if constexpr (V == 2) {
+            if (aligned && base + 2 <= n) {
+                const typename Vec16<double>::type w = *(const typename Vec16<double>::type *)(p + base);
+                const double w0 = w[0], w1 = w[1];
+                x[0] = {w0, w0};
+                x[1] = {w1, w1};
+                return;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            if (base + k < n) {
+                const double v = p[base + k];
+                x[k] = {v, v};
+            } else {
+                x[k] = Op::neutral();
+            }
+        }
+    }
+};
+template <> struct io_dual<PairIn> { static constexpr bool value = true; };
+
 // One kernel for: unsegmented exclusive scan (SEG=false, REDUCE=false),
-// segmented exclusive scan (SEG=true, REDUCE=false) and segmented reduce
-// (SEG=true, REDUCE=true; `out` has one element per segment).
+// segmented exclusive scan (SEG=true, REDUCE=false), segmented reduce
+// (SEG=true, REDUCE=true; `out` has one element per segment) and, with a
+// dual loader (SEG=true, REDUCE=true), both at once on a pair.
 template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>>
 __global__ __launch_bounds__(kBlock) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                                  uint64_t *__restrict__ status, unsigned *__restrict__ counter,
@@ -114,6 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(In in, typename Op::T *__restri
     using St = TileStatus<T>;
     constexpr int V = S::V;
     constexpr uint32_t VMASK = (1u << V) - 1u;
+    constexpr bool DUAL = io_dual<In>::value;
+    static_assert(!DUAL || (SEG && REDUCE), "a dual pass is a segmented scan and reduction");
     const Op op;
     const bool aligned = aligned_i != 0;
 
@@ -194,7 +258,11 @@ __global__ __launch_bounds__(kBlock) void k_scan(In in, typename Op::T *__restri
         for (int k = 0; k < V; k++) {
             const bool h = SEG && ((hb[r] >> k) & 1u);
             T y;
-            if constexpr (!REDUCE) {
+            if constexpr (DUAL) {             // a: exclusive, b: inclusive; same `needs`
+                if (h) { y.a = Op::neutral().a; p = x[r][k]; pf = 1; }
+                else { y.a = p.a; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
+                y.b = p.b;
+            } else if constexpr (!REDUCE) {
                 if (h) { y = Op::neutral(); p = x[r][k]; pf = 1; }
                 else { y = p; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
             } else {
@@ -258,7 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_scan(In in, typename Op::T *__restri
             if ((needs >> (r * V + k)) & 1ull) v = op(inc, v);
             y[k] = op(cid, v);
         }
-        if constexpr (!REDUCE) {
+        if constexpr (DUAL) {
+#pragma unroll
+            for (int k = 0; k < V; k++) {
+                const int64_t e = base + k;
+                if (e < n) in.scan[e] = y[k].a;
+                if ((eb[r] >> k) & 1u) {
+                    const uint32_t wd = sb.heads[e >> 5];
+                    const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);
+                    const int rk = sb.wcnt[e >> 5] + __popc(wd & mask);
+                    in.red[sb.nz[rk - 1]] = y[k].b;
+                }
+            }
+        } else if constexpr (!REDUCE) {
             store_lane<T, V>(out, base, n, aligned, y);
         } else {
 #pragma unroll
