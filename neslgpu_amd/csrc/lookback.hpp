@@ -110,15 +110,23 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
             v[j] = Op::neutral();
             f[j] = t >= 0 ? kInvalid : kPrefix;
         }
+        // Poll until every tile nearer than the nearest full prefix in the
+        // window has published (tiles beyond that prefix do not matter, so
+        // a straggler there is not waited for); with no prefix in the
+        // window, until all of it has.
         for (int spins = 0;; spins++) {
-            bool waiting = false;
+            int pj = K, bj = K;          // nearest prefix / nearest unpublished tile of this lane
 #pragma unroll
-            for (int j = 0; j < K; j++) {
+            for (int j = K - 1; j >= 0; j--) {
                 const int64_t t = pos - (int64_t)lane * K - j;
                 if (f[j] == kInvalid) f[j] = St::read(st, t, v[j]);
-                waiting |= f[j] == kInvalid;
+                if (f[j] == kPrefix) pj = j;
+                if (f[j] == kInvalid) bj = j;
             }
-            if (ballot(waiting) == 0) break;
+            const uint64_t pmask = ballot(pj < K);
+            const int PL = pmask ? (int)__builtin_ctzll(pmask) : kWave;
+            const bool blocks = lane < PL ? bj < K : (lane == PL && bj < pj);
+            if (ballot(blocks) == 0) break;
             if (spins >= kSpinLimit) {
 #pragma unroll
                 for (int j = 0; j < K; j++)
