@@ -218,18 +218,63 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     unsigned loc[kRankItems];
     const uint64_t lt = (lane == 0) ? 0 : (~0ull >> (64 - lane));
 
+    // Item j of a lane is element wb + j*64 + lane: each round of the
+    // multisplit below holds 64 consecutive elements, which keeps it stable.
+    // A whole span of 4-byte keys is read with 16-byte loads (lane l gets
+    // elements 4l..4l+3 of each 256) and put in that order through the
+    // wave's own part of s_stage, which is free until the staging phase.
+    if constexpr (sizeof(KeyT) == 4) {
+        if (a.vec4 && wb + kWaveSpan <= a.n) {
+            static_assert(kRankItems % 4 == 0, "runs of four keys per lane");
+            uint32_t *sw = (uint32_t *)s_stage + (int64_t)w * kWaveSpan;
+            auto transpose = [&](const uint32_t *src, uint32_t (&out)[kRankItems]) {
+#pragma unroll
+                for (int g = 0; g < kRankItems / 4; g++) {
+                    const u32x4 v = *(const u32x4 *)(src + wb + g * 4 * kWave + 4 * lane);
+                    *(u32x4 *)(sw + g * 4 * kWave + 4 * lane) = v;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int j = 0; j < kRankItems; j++) out[j] = sw[j * kWave + lane];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            };
+            uint32_t raw[kRankItems];
+            transpose((const uint32_t *)a.src, raw);
+#pragma unroll
+            for (int j = 0; j < kRankItems; j++) key[j] = (KeyT)(a.source == 0 ? int_key((int)raw[j], a.up) : raw[j]);
+            if (a.idx_in) {
+                transpose((const uint32_t *)a.idx_in, raw);
+#pragma unroll
+                for (int j = 0; j < kRankItems; j++) idx[j] = (int)raw[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < kRankItems; j++) idx[j] = (int)(wb + (int64_t)j * kWave + lane);
+            }
+            goto loaded;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kRankItems; j++) {
         const int64_t e = wb + (int64_t)j * kWave + lane;
-        const bool in = e < a.n;
         KeyT k = 0;
         int ix = 0;
-        if (in) {
+        if (e < a.n) {
             ix = a.idx_in ? a.idx_in[e] : (int)e;
             k = pass_key<KeyT>(a, e, ix);
         }
         key[j] = k;
         idx[j] = ix;
+    }
+loaded:
+#pragma unroll
+    for (int j = 0; j < kRankItems; j++) {
+        const int64_t e = wb + (int64_t)j * kWave + lane;
+        const bool in = e < a.n;
+        const KeyT k = key[j];
         const unsigned d = in ? (unsigned)((k >> a.shift) & 255) : 0u;
         dig[j] = d;
         // lanes of this round holding the same digit (inactive lanes excluded)
