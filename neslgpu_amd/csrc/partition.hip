@@ -171,9 +171,15 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     __shared__ unsigned s_hist[kMaxFan];
     __shared__ unsigned s_start[kMaxFan];
     __shared__ unsigned s_gofs[kMaxFan];
-    __shared__ int s_idx[kPartTile];
-    __shared__ T s_val[kPartTile];
-    __shared__ unsigned char s_dig[kPartTile];
+    // The tile staged in bucket order: four-byte values share one 8-byte
+    // word with their index (one LDS access each way, and the bucket is
+    // recomputed from the index); wider values keep the index, the value
+    // and the bucket apart.
+    constexpr bool PACKED = sizeof(T) == 4;
+    __shared__ uint2 s_pair[PACKED ? kPartTile : 1];
+    __shared__ int s_idx[PACKED ? 1 : kPartTile];
+    __shared__ T s_val[PACKED ? 1 : kPartTile];
+    __shared__ unsigned char s_dig[PACKED ? 1 : kPartTile];
 
     const int tid = threadIdx.x;
     const int64_t bt = xcd_tile(blockIdx.x, gridDim.x);
@@ -276,22 +282,38 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     for (int j = 0; j < kPartItems; j++) {
         if (dg[j] >= 0) {
             const unsigned at = s_start[dg[j]] + rk[j];
-            s_idx[at] = q[j];
-            s_val[at] = v[j];
-            s_dig[at] = (unsigned char)dg[j];
+            if constexpr (PACKED) {
+                s_pair[at] = make_uint2((unsigned)q[j], elem_as<unsigned>(v[j]));
+            } else {
+                s_idx[at] = q[j];
+                s_val[at] = v[j];
+                s_dig[at] = (unsigned char)dg[j];
+            }
         }
     }
     __syncthreads();
     const int64_t total = (int64_t)s_start[kMaxFan - 1] + s_hist[kMaxFan - 1];
     const int64_t csize = (int64_t)1 << clog;
     for (int64_t i = tid; i < total; i += kPartThreads) {
-        const int d = s_dig[i];
+        int qi;
+        T vi;
+        int d;
+        if constexpr (PACKED) {
+            const uint2 w = s_pair[i];
+            qi = (int)w.x;
+            vi = __builtin_bit_cast(T, w.y);
+            d = (int)(((int64_t)qi >> clog) - first_child);   // in [0, 256): checked when ranked
+        } else {
+            qi = s_idx[i];
+            vi = s_val[i];
+            d = s_dig[i];
+        }
         const int64_t child = first_child + d;
         const int64_t slot = (int64_t)s_gofs[d] + (i - s_start[d]);
         const int64_t cend = (child + 1) * csize < n ? (child + 1) * csize : n;
         const int64_t pos = child * csize + slot;
         if (pos < cend) {
-            out.store(pos, s_idx[i], s_val[i]);
+            out.store(pos, qi, vi);
         } else {
             atomicOr(err, 1u);   // more pairs than the block holds: not a permutation
         }
