@@ -298,4 +298,33 @@ int64_t resident_grid(const void *kernel, int64_t work);
 // ------------------------------------------------------------------ fills
 void fill_u32(void *p, uint32_t v, int64_t count);   // stream-ordered fill kernel
 
+// ------------------------------------------------------------------ scalar results to the host
+// A CVL call that returns a scalar (a reduction, ext_vu*) has its kernel
+// post the value into a word pair of pinned, device-mapped host memory;
+// the host waits on the sequence number instead of copying the value out
+// and synchronizing the stream.  Posts are ordered: the value is written
+// before the sequence number, both at system scope.
+struct Mailbox {
+    unsigned long long seq;
+    unsigned long long value;
+};
+Mailbox *mailbox_device();                              // device address (allocated on first use)
+unsigned long long mailbox_next();                      // sequence number of the next post
+unsigned long long mailbox_wait(unsigned long long seq);   // host: the value posted with seq
+
+__device__ __forceinline__ void mailbox_post(Mailbox *mb, unsigned long long seq, unsigned long long bits) {
+    __hip_atomic_store(&mb->value, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T>
+__host__ __device__ __forceinline__ unsigned long long scalar_bits(T v) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(unsigned long long, v);
+    else return (unsigned long long)__builtin_bit_cast(unsigned, v);
+}
+template <class T>
+__host__ __device__ __forceinline__ T scalar_from_bits(unsigned long long b) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, b);
+    else return __builtin_bit_cast(T, (unsigned)b);
+}
+
 }  // namespace cvlhip

@@ -17,7 +17,6 @@
 
 namespace cvlhip {
 
-static void *g_ext_host = nullptr;   // pinned landing slot of scalar extracts
 
 static int ew_grid(int64_t n) {
     int g = grid_for(n, kBlock * 4);
@@ -54,6 +53,12 @@ __global__ void k_sel(T *__restrict__ d, const int *__restrict__ f, const T *__r
 template <class T>
 __global__ void k_put1(T *p, T v) {
     if (threadIdx.x == 0) *p = v;
+}
+
+// One element to the host mailbox (ext_vu*).
+template <class T>
+__global__ void k_ext_post(const T *p, Mailbox *mb, unsigned long long seq) {
+    if (threadIdx.x == 0) mailbox_post(mb, seq, scalar_bits(*p));
 }
 
 template <class T>
@@ -341,17 +346,19 @@ int ind_lez_scratch(int n, int m) { (void)n; (void)m; return 0; }
 unsigned int ind_lez_inplace(void) { return INPLACE_NONE; }
 
 // ---- extract / replace of one element
-// EXT reads the element into pinned host memory and waits for it (the caller
-// needs the value); REP writes it with a one-lane kernel that takes the value
-// as an argument, so it queues behind earlier work without a host wait.
+// EXT has a one-lane kernel post the element to the host mailbox and waits
+// for it (the caller needs the value); REP writes it with a one-lane kernel
+// that takes the value as an argument, so it queues behind earlier work
+// without a host wait.
 #define EXT(NAME, T)                                                                           \
     T NAME(vec_p V, int i, int len, vec_p scratch) {                                           \
         (void)scratch;                                                                         \
         if (i < 0 || i >= len) { usage_error(#NAME ": index %d outside [0,%d)", i, len); return (T)0; } \
-        if (!g_ext_host) HIPCHECK(hipHostMalloc(&g_ext_host, 64, hipHostMallocDefault));      \
-        HIPCHECK(hipMemcpyAsync(g_ext_host, (const T *)V + i, sizeof(T), hipMemcpyDeviceToHost, g_stream)); \
-        HIPCHECK(hipStreamSynchronize(g_stream));                                              \
-        return *(const T *)g_ext_host;                                                         \
+        Mailbox *mb = mailbox_device();                                                        \
+        const unsigned long long seq = mailbox_next();                                         \
+        k_ext_post<T><<<1, 64, 0, g_stream>>>((const T *)V + i, mb, seq);                      \
+        LAUNCHCHECK();                                                                         \
+        return scalar_from_bits<T>(mailbox_wait(seq));                                         \
     }                                                                                          \
     NOSCR(NAME, INPLACE_NONE)
 EXT(ext_vuz, int)

@@ -80,6 +80,44 @@ int64_t resident_grid(const void *kernel, int64_t work) {
     return g < work ? g : (work > 0 ? work : 1);
 }
 
+static Mailbox *g_mb_host = nullptr;
+static Mailbox *g_mb_dev = nullptr;
+static unsigned long long g_mb_seq = 0;
+
+Mailbox *mailbox_device() {
+    if (!g_mb_host) {
+        HIPCHECK(hipHostMalloc((void **)&g_mb_host, 256,
+                               hipHostMallocMapped | hipHostMallocCoherent | hipHostMallocPortable));
+        g_mb_host->seq = 0;
+        g_mb_host->value = 0;
+        HIPCHECK(hipHostGetDevicePointer((void **)&g_mb_dev, g_mb_host, 0));
+    }
+    return g_mb_dev;
+}
+
+unsigned long long mailbox_next() { return ++g_mb_seq; }
+
+// Spins on the sequence number for up to 2 ms, then waits for the stream
+// (which reports a failed kernel) and looks once more.
+unsigned long long mailbox_wait(unsigned long long seq) {
+    const struct timespec t0 = [] { struct timespec t; clock_gettime(CLOCK_MONOTONIC, &t); return t; }();
+    for (unsigned it = 1;; it++) {
+        if (__atomic_load_n(&g_mb_host->seq, __ATOMIC_ACQUIRE) == seq)
+            return __atomic_load_n(&g_mb_host->value, __ATOMIC_RELAXED);
+        if ((it & 1023) == 0) {
+            struct timespec t;
+            clock_gettime(CLOCK_MONOTONIC, &t);
+            if ((t.tv_sec - t0.tv_sec) * 1000000000LL + (t.tv_nsec - t0.tv_nsec) > 2000000LL) {
+                HIPCHECK(hipStreamSynchronize(g_stream));
+                if (__atomic_load_n(&g_mb_host->seq, __ATOMIC_ACQUIRE) == seq)
+                    return __atomic_load_n(&g_mb_host->value, __ATOMIC_RELAXED);
+                fatal("a kernel finished without posting its scalar result (sequence %llu)", seq);
+            }
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 void fill_u32(void *p, uint32_t v, int64_t count) {
     if (count <= 0) return;
     int g = grid_for(count, kBlock * 4);

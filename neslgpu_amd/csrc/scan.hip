@@ -29,7 +29,7 @@ template <class Op, int R>
 __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restrict__ in, int64_t n,
                                                    int64_t per_block, typename Op::T *__restrict__ partials,
                                                    unsigned *__restrict__ counter, typename Op::T *__restrict__ result,
-                                                   int aligned_i) {
+                                                   int aligned_i, Mailbox *mb, unsigned long long seq) {
     using T = typename Op::T;
     using S = ScanShape<Op, R>;
     constexpr int V = S::V;
@@ -84,7 +84,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restr
                                      : Op::neutral();
             tot = op(tot, wave_reduce(v, op));
         }
-        if (lane == 0) result[0] = op(Op::cvl_id(), tot);
+        if (lane == 0) {
+            const T r = op(Op::cvl_id(), tot);
+            result[0] = r;
+            if (mb) mailbox_post(mb, seq, scalar_bits(r));
+        }
     }
 }
 
@@ -150,7 +154,8 @@ template <class Op, int U>
 __global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T *__restrict__ in, int64_t n,
                                                            typename Op::T *__restrict__ partials,
                                                            unsigned *__restrict__ counter,
-                                                           typename Op::T *__restrict__ result, int aligned_i) {
+                                                           typename Op::T *__restrict__ result, int aligned_i,
+                                                           Mailbox *mb, unsigned long long seq) {
     using T = typename Op::T;
     using S = ScanShape<Op, 1>;
     constexpr int V = S::V;
@@ -194,7 +199,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T 
                                      : Op::neutral();
             tot = op(tot, wave_reduce(v, op));
         }
-        if (lane == 0) result[0] = op(Op::cvl_id(), tot);
+        if (lane == 0) {
+            const T r = op(Op::cvl_id(), tot);
+            result[0] = r;
+            if (mb) mailbox_post(mb, seq, scalar_bits(r));
+        }
     }
 }
 
@@ -212,10 +221,11 @@ int64_t reduce_scratch_bytes(int64_t n) {
     return 256 + 1024 * 8 + 256;
 }
 
-static void *g_host_result = nullptr;
-
+// The reduction of s into result_dev (device memory); with a mailbox, the
+// kernel also posts it to the host under `seq`.
 template <class Op>
-void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch) {
+void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch, Mailbox *mb = nullptr,
+                      unsigned long long seq = 0) {
     using T = typename Op::T;
     using S = ScanShape<Op, kScanRows>;
     if (!scratch) usage_error("reduction called without scratch");
@@ -234,7 +244,7 @@ void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch)
         const int grid = (int)(want < CVLHIP_REDUCE_GRID ? want : CVLHIP_REDUCE_GRID);
         if (grid > 1) fill_u32(counter, 0, 1);
         k_reduce_strided<Op, CVLHIP_REDUCE_U><<<grid, kBlock, 0, g_stream>>>(
-            (const T *)s, n, partials, grid > 1 ? counter : nullptr, (T *)result_dev, al16(s));
+            (const T *)s, n, partials, grid > 1 ? counter : nullptr, (T *)result_dev, al16(s), mb, seq);
         LAUNCHCHECK();
         return;
 #endif
@@ -246,20 +256,21 @@ void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch)
     grid = (int)((n + per_block - 1) / per_block);
     if (grid > 1) fill_u32(counter, 0, 1);
     k_reduce<Op, kScanRows><<<grid, kBlock, 0, g_stream>>>((const T *)s, n, per_block, partials,
-                                                           grid > 1 ? counter : nullptr, (T *)result_dev, al16(s));
+                                                           grid > 1 ? counter : nullptr, (T *)result_dev, al16(s),
+                                                           mb, seq);
     LAUNCHCHECK();
 }
 
 template <class Op>
 typename Op::T reduce_unseg(const void *s, int64_t n, void *scratch) {
     using T = typename Op::T;
-    if (!g_host_result) HIPCHECK(hipHostMalloc(&g_host_result, 64, hipHostMallocDefault));
     if (n <= 0) return Op::cvl_id();
+    if (!scratch) usage_error("reduction called without scratch");
     T *res = (T *)((char *)scratch + 256 + 1024 * 8);
-    reduce_unseg_dev<Op>(res, s, n, scratch);
-    HIPCHECK(hipMemcpyAsync(g_host_result, res, sizeof(T), hipMemcpyDeviceToHost, g_stream));
-    HIPCHECK(hipStreamSynchronize(g_stream));
-    return *(T *)g_host_result;
+    Mailbox *mb = mailbox_device();
+    const unsigned long long seq = mailbox_next();
+    reduce_unseg_dev<Op>(res, s, n, scratch, mb, seq);
+    return scalar_from_bits<T>(mailbox_wait(seq));
 }
 
 template <class Op>
