@@ -31,10 +31,6 @@ void cvlhip_synchronize(void);
 /* Device address of the error word kernels raise (generated fused
  * operations report into it too); read it with cvlhip_check_device_errors. */
 void *cvlhip_device_error_word(void);
-/* Grid of the tile kernels (scan, pack): cap > 0 keeps at most `cap`
- * resident blocks per compute unit, each looping over tiles; 0 = as many
- * resident blocks as the occupancy allows; -1 = one block per tile. */
-void cvlhip_set_blocks_per_cu(int cap);
 
 /* ---- segment descriptors ------------------------------------------------
  * Byte offset of the m int32 segment lengths inside a descriptor built by

@@ -289,11 +289,6 @@ __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Number of blocks of `kernel` (kBlock threads) that fit on the device at
-// once, capped at `work` and at the blocks-per-CU limit set by
-// cvlhip_set_blocks_per_cu (0 = as many as fit).  Persistent kernels use it
-// as their grid.
-int64_t resident_grid(const void *kernel, int64_t work);
 
 // ------------------------------------------------------------------ fills
 void fill_u32(void *p, uint32_t v, int64_t count);   // stream-ordered fill kernel

@@ -62,24 +62,6 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, int64_t count) 
     for (; i < count; i += stride) p[i] = v;
 }
 
-static int g_cus = 0;
-static int g_blocks_per_cu_cap = 0;
-
-int64_t resident_grid(const void *kernel, int64_t work) {
-    if (g_cus == 0) {
-        int dev = 0;
-        HIPCHECK(hipGetDevice(&dev));
-        HIPCHECK(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev));
-    }
-    if (g_blocks_per_cu_cap < 0) return work > 0 ? work : 1;   // one tile per block
-    int per_cu = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    if (g_blocks_per_cu_cap > 0 && per_cu > g_blocks_per_cu_cap) per_cu = g_blocks_per_cu_cap;
-    int64_t g = (int64_t)per_cu * g_cus;
-    return g < work ? g : (work > 0 ? work : 1);
-}
-
 static Mailbox *g_mb_host = nullptr;
 static Mailbox *g_mb_dev = nullptr;
 static unsigned long long g_mb_seq = 0;
@@ -236,7 +218,6 @@ void cvlhip_graph_destroy(void *exec) {
     if (exec) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
 }
 
-void cvlhip_set_blocks_per_cu(int cap) { g_blocks_per_cu_cap = cap; }
 
 // ------------------------------------------------------------------ CVL facilities
 void CVL_init(char *fused_library) {
