@@ -51,6 +51,8 @@ def parse():
     p.add_argument("--irregular-log2n", type=int, default=26)
     p.add_argument("--no-programs", action="store_true",
                    help="skip the whole-program workloads (quicksort, quickhull, SpMV through vinterp)")
+    p.add_argument("--programs", default="",
+                   help="comma list of program workloads to run alone (profiling), e.g. quickhull_fused,spmv_fused")
     return p.parse_args()
 
 
@@ -298,13 +300,16 @@ def program_timer(stdout: str) -> float:
     return float(stdout.splitlines()[-1].strip("() "))
 
 
-def programs(reps: int) -> dict:
+def programs(reps: int, only=()) -> dict:
     """Whole VCODE programs through vinterp on libcvl_hip.so: median of the
     programs' own START_TIMER / STOP_TIMER interval (data generation
     excluded), in ms.  SpMV repeats the product 10 times inside the timed
-    interval; its figures are per product."""
+    interval; its figures are per product.  `only`: names to run (all when
+    empty)."""
     from neslgpu_amd import vinterp as V
     work, rows = program_workloads()
+    if only:
+        work = [w for w in work if w[0] in only]
     out = {}
     nnz, m, ncols = int(rows.sum()), int(rows.size), 1 << 22
     for name, text, stdin, fused, graphs, k, desc in work:
@@ -510,7 +515,9 @@ def main():
     }
     if not a.no_irregular and not a.only:
         out["irregular_f64"] = irregular(a.irregular_log2n, max(3, a.steps))
-    if not a.no_programs and not a.only:
+    if a.programs:
+        out["programs"] = programs(3, tuple(a.programs.split(",")))
+    elif not a.no_programs and not a.only:
         out["programs"] = programs(3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
