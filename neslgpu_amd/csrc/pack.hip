@@ -173,7 +173,7 @@ using namespace cvlhip;
 extern "C" {
 
 int pk1_luv(vec_p f, int len, vec_p scratch) { return reduce_unseg<CountB>(f, len, scratch); }
-int pk1_luv_scratch(int len) { return units_for_bytes(256 + 1024 * 8 + 256); (void)len; }
+int pk1_luv_scratch(int len) { return units_for_bytes(kReduceScratchBytes); (void)len; }
 unsigned int pk1_luv_inplace(void) { return INPLACE_NONE; }
 
 void pk1_lev(vec_p ds, vec_p f, vec_p sd, int n, int m, vec_p scratch) {
@@ -181,7 +181,7 @@ void pk1_lev(vec_p ds, vec_p f, vec_p sd, int n, int m, vec_p scratch) {
 }
 int pk1_lev_scratch(int n, int m) {
     (void)m;
-    int64_t a = scan_scratch_bytes<AddZ>(n), b = 256 + 1024 * 8 + 256;
+    int64_t a = scan_scratch_bytes<AddZ>(n), b = kReduceScratchBytes;
     return units_for_bytes(a > b ? a : b);
 }
 unsigned int pk1_lev_inplace(void) { return INPLACE_NONE; }

@@ -20,23 +20,68 @@
 
 namespace cvlhip {
 
+// The block partials of a reduction are combined by k_reduce_finish, a
+// second launch of one wave, in block order.  Combining them in the last
+// block to finish would need an acquire-release counter in every block,
+// which on MI355X writes back and invalidates the XCD's L2 each time: at
+// 1024 blocks a 1 GiB int32 reduction then reads at 4.8 TB/s instead of
+// 6.0 (6.3 at 2048 blocks without it).
+
+// Result of a reduction: fold the g block partials in block order, then
+// the CVL identity (as the serial loop starts from it), store it and post
+// it to the host when asked.
+template <class Op>
+__global__ __launch_bounds__(kWave) void k_reduce_finish(const typename Op::T *__restrict__ partials, int g,
+                                                         typename Op::T *__restrict__ result, Mailbox *mb,
+                                                         unsigned long long seq) {
+    using T = typename Op::T;
+    const Op op;
+    const int lane = threadIdx.x;
+    T tot = Op::neutral();
+    for (int i0 = 0; i0 < g; i0 += kWave) {
+        const int i = i0 + lane;
+        const T v = i < g ? partials[i] : Op::neutral();
+        tot = op(tot, wave_reduce(v, op));
+    }
+    if (lane == 0) {
+        const T r = op(Op::cvl_id(), tot);
+        result[0] = r;
+        if (mb) mailbox_post(mb, seq, scalar_bits(r));
+    }
+}
+
+// A block's partial: to `partials` when the grid has several blocks, else
+// (one block holds the whole reduction) straight to the result.
+template <class Op>
+__device__ __forceinline__ void reduce_block_out(typename Op::T b, typename Op::T *__restrict__ partials,
+                                                 typename Op::T *__restrict__ result, Mailbox *mb,
+                                                 unsigned long long seq) {
+    using T = typename Op::T;
+    if (gridDim.x > 1) {
+        partials[blockIdx.x] = b;
+    } else {
+        const T r = Op()(Op::cvl_id(), b);
+        result[0] = r;
+        if (mb) mailbox_post(mb, seq, scalar_bits(r));
+    }
+}
+
 // Unsegmented reduction.  Block b folds a fixed contiguous range (the
 // partition depends only on n), each wave a contiguous quarter of it, rows
-// in order; partials are combined in block order by the last block to
-// finish.  The result is therefore deterministic, and a left fold for the
+// in order; k_reduce_finish combines the partials in block order.  The
+// result is therefore deterministic, and a left fold for the
 // order-sensitive double max/min.
 template <class Op, int R>
 __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restrict__ in, int64_t n,
                                                    int64_t per_block, typename Op::T *__restrict__ partials,
-                                                   unsigned *__restrict__ counter, typename Op::T *__restrict__ result,
-                                                   int aligned_i, Mailbox *mb, unsigned long long seq) {
+                                                   typename Op::T *__restrict__ result, int aligned_i, Mailbox *mb,
+                                                   unsigned long long seq) {
     using T = typename Op::T;
     using S = ScanShape<Op, R>;
     constexpr int V = S::V;
     const Op op;
     const bool aligned = aligned_i != 0;
     __shared__ T s_w[S::WAVES];
-    __shared__ bool s_last;
 
     const int w = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -67,28 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const typename Op::T *__restr
     if (threadIdx.x == 0) {
         T b = s_w[0];
         for (int i = 1; i < S::WAVES; i++) b = op(b, s_w[i]);
-        __hip_atomic_store(partials + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (counter) {
-            unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            s_last = prev == gridDim.x - 1;
-        } else {
-            s_last = true;                  // a one-block launch finishes alone
-        }
-    }
-    __syncthreads();
-    if (s_last && w == 0) {
-        T tot = Op::neutral();
-        for (int i0 = 0; i0 < (int)gridDim.x; i0 += kWave) {
-            const int i = i0 + lane;
-            T v = i < (int)gridDim.x ? __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : Op::neutral();
-            tot = op(tot, wave_reduce(v, op));
-        }
-        if (lane == 0) {
-            const T r = op(Op::cvl_id(), tot);
-            result[0] = r;
-            if (mb) mailbox_post(mb, seq, scalar_bits(r));
-        }
+        reduce_block_out<Op>(b, partials, result, mb, seq);
     }
 }
 
@@ -153,7 +177,6 @@ void scan_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void
 template <class Op, int U>
 __global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T *__restrict__ in, int64_t n,
                                                            typename Op::T *__restrict__ partials,
-                                                           unsigned *__restrict__ counter,
                                                            typename Op::T *__restrict__ result, int aligned_i,
                                                            Mailbox *mb, unsigned long long seq) {
     using T = typename Op::T;
@@ -162,7 +185,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T 
     const Op op;
     const bool aligned = aligned_i != 0;
     __shared__ T s_w[S::WAVES];
-    __shared__ bool s_last;
     const int w = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int64_t nw = (int64_t)gridDim.x * S::WAVES;
@@ -182,43 +204,25 @@ __global__ __launch_bounds__(kBlock) void k_reduce_strided(const typename Op::T 
     if (threadIdx.x == 0) {
         T b = s_w[0];
         for (int i = 1; i < S::WAVES; i++) b = op(b, s_w[i]);
-        __hip_atomic_store(partials + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (counter) {
-            unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            s_last = prev == gridDim.x - 1;
-        } else {
-            s_last = true;
-        }
-    }
-    __syncthreads();
-    if (s_last && w == 0) {
-        T tot = Op::neutral();
-        for (int i0 = 0; i0 < (int)gridDim.x; i0 += kWave) {
-            const int i = i0 + lane;
-            T v = i < (int)gridDim.x ? __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : Op::neutral();
-            tot = op(tot, wave_reduce(v, op));
-        }
-        if (lane == 0) {
-            const T r = op(Op::cvl_id(), tot);
-            result[0] = r;
-            if (mb) mailbox_post(mb, seq, scalar_bits(r));
-        }
+        reduce_block_out<Op>(b, partials, result, mb, seq);
     }
 }
 
 #ifndef CVLHIP_REDUCE_GRID
-#define CVLHIP_REDUCE_GRID 1024
+#define CVLHIP_REDUCE_GRID 2048
 #endif
+// One row per wave per step.  Unrolling steps (several rows in flight per
+// wave, each a grid-width apart) reads slower: 1 GiB of int32 at grid 1024
+// reads at 6.2 TB/s with one row per step and 5.0 TB/s with eight.
 #ifndef CVLHIP_REDUCE_U
-#define CVLHIP_REDUCE_U 8
+#define CVLHIP_REDUCE_U 1
 #endif
-static_assert(CVLHIP_REDUCE_GRID <= 1024, "the reduction scratch holds 1024 block partials");
+static_assert(CVLHIP_REDUCE_GRID <= kReducePartials, "the reduction scratch holds kReducePartials partials");
 
 template <class Op>
 int64_t reduce_scratch_bytes(int64_t n) {
     (void)n;
-    return 256 + 1024 * 8 + 256;
+    return kReduceScratchBytes;
 }
 
 // The reduction of s into result_dev (device memory); with a mailbox, the
@@ -229,36 +233,34 @@ void reduce_unseg_dev(void *result_dev, const void *s, int64_t n, void *scratch,
     using T = typename Op::T;
     using S = ScanShape<Op, kScanRows>;
     if (!scratch) usage_error("reduction called without scratch");
-    unsigned *counter = (unsigned *)scratch;
     T *partials = (T *)((char *)scratch + 256);
     if (n <= 0) {
         k_set1<T><<<1, 64, 0, g_stream>>>((T *)result_dev, Op::cvl_id());
         LAUNCHCHECK();
         return;
     }
+    int grid;
     if constexpr (!ordered<Op>::value) {
-#ifndef CVLHIP_REDUCE_CHUNKED
         using S1 = ScanShape<Op, 1>;
         constexpr int64_t step = (int64_t)S1::WAVES * S1::ROW * CVLHIP_REDUCE_U;   // elements per block per step
         const int64_t want = (n + step - 1) / step;
-        const int grid = (int)(want < CVLHIP_REDUCE_GRID ? want : CVLHIP_REDUCE_GRID);
-        if (grid > 1) fill_u32(counter, 0, 1);
+        grid = (int)(want < CVLHIP_REDUCE_GRID ? want : CVLHIP_REDUCE_GRID);
         k_reduce_strided<Op, CVLHIP_REDUCE_U><<<grid, kBlock, 0, g_stream>>>(
-            (const T *)s, n, partials, grid > 1 ? counter : nullptr, (T *)result_dev, al16(s), mb, seq);
-        LAUNCHCHECK();
-        return;
-#endif
+            (const T *)s, n, partials, (T *)result_dev, al16(s), mb, seq);
+    } else {
+        const int64_t tiles = (n + S::TILE - 1) / S::TILE;
+        grid = (int)(tiles < kReducePartials ? tiles : kReducePartials);
+        int64_t per_block = (n + grid - 1) / grid;
+        per_block = (per_block + S::WAVES * S::ROW - 1) / (S::WAVES * S::ROW) * (S::WAVES * S::ROW);
+        grid = (int)((n + per_block - 1) / per_block);
+        k_reduce<Op, kScanRows><<<grid, kBlock, 0, g_stream>>>((const T *)s, n, per_block, partials,
+                                                               (T *)result_dev, al16(s), mb, seq);
     }
-    int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    int grid = (int)(tiles < 1024 ? tiles : 1024);
-    int64_t per_block = (n + grid - 1) / grid;
-    per_block = (per_block + S::WAVES * S::ROW - 1) / (S::WAVES * S::ROW) * (S::WAVES * S::ROW);
-    grid = (int)((n + per_block - 1) / per_block);
-    if (grid > 1) fill_u32(counter, 0, 1);
-    k_reduce<Op, kScanRows><<<grid, kBlock, 0, g_stream>>>((const T *)s, n, per_block, partials,
-                                                           grid > 1 ? counter : nullptr, (T *)result_dev, al16(s),
-                                                           mb, seq);
     LAUNCHCHECK();
+    if (grid > 1) {
+        k_reduce_finish<Op><<<1, kWave, 0, g_stream>>>(partials, grid, (T *)result_dev, mb, seq);
+        LAUNCHCHECK();
+    }
 }
 
 template <class Op>
@@ -266,7 +268,7 @@ typename Op::T reduce_unseg(const void *s, int64_t n, void *scratch) {
     using T = typename Op::T;
     if (n <= 0) return Op::cvl_id();
     if (!scratch) usage_error("reduction called without scratch");
-    T *res = (T *)((char *)scratch + 256 + 1024 * 8);
+    T *res = (T *)((char *)scratch + kReduceResultOffset);
     Mailbox *mb = mailbox_device();
     const unsigned long long seq = mailbox_next();
     reduce_unseg_dev<Op>(res, s, n, scratch, mb, seq);

@@ -13,4 +13,10 @@ template <class Op> void reduce_seg(void *d, const void *s, const void *sd, int6
 template <class Op> typename Op::T reduce_unseg(const void *s, int64_t n, void *scratch);
 template <class Op> int64_t scan_scratch_bytes(int64_t n);
 
+// Scratch of an unsegmented reduction: a 256-byte header, one 8-byte slot
+// per block partial, and the result slot (at kReduceResultOffset).
+constexpr int kReducePartials = 2048;     // most blocks of one reduction
+constexpr int64_t kReduceResultOffset = 256 + (int64_t)kReducePartials * 8;
+constexpr int64_t kReduceScratchBytes = kReduceResultOffset + 256;
+
 }  // namespace cvlhip

@@ -253,7 +253,7 @@ void cvlhip_seg_reduce(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int 
 
 int cvlhip_seg_scratch(int n, int m) {
     (void)m;
-    int64_t a = scan_scratch_bytes<AddD>(n) * 2, b = 256 + 1024 * 8 + 256;
+    int64_t a = scan_scratch_bytes<AddD>(n) * 2, b = kReduceScratchBytes;
     return units_for_bytes(a > b ? a : b);
 }
 
