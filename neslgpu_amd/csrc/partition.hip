@@ -59,6 +59,7 @@ constexpr int kPartV = 4;
 static_assert(kPartItems % kPartV == 0, "items per thread come in runs of kPartV");
 constexpr int kMaxFan = 256;                   // children per region per pass
 static_assert(kMaxFan == kTileScanCols, "first-pass histogram rows are scanned 256 wide");
+static_assert(kBlkLog4 <= 16 && kBlkLog8 <= 16, "offsets within a final block fit the 16 bits the last pass keeps");
 
 // Where the destination index and the value of input element k come from
 // on the first pass: the index is p[k] (plain permute) or offs[seg(k)] + p[k]
@@ -120,6 +121,7 @@ template <class T, bool PACKED = sizeof(T) == 4>
 struct Pairs {
     int *idx;
     T *val;
+    static constexpr bool LOCAL = false;
     __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
         q = idx[e];
         v = val[e];
@@ -137,6 +139,7 @@ struct Pairs {
 template <class T>
 struct Pairs<T, true> {
     uint2 *p;
+    static constexpr bool LOCAL = false;
     __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
         const uint2 w = p[e];
         q = (int)w.x;
@@ -154,14 +157,41 @@ struct Pairs<T, true> {
     }
 };
 
+// Output of the last partition pass, read by the block scatter: the value
+// and the destination index's low 16 bits, whose low blk_log<T>() bits are
+// the offset within the destination block (a block starts at a multiple of
+// its size).  Six bytes per four-byte value instead of eight.
+template <class T>
+struct Local16 {
+    unsigned short *idx;
+    T *val;
+    static constexpr bool LOCAL = true;
+    __device__ __forceinline__ void load(int64_t e, int &q, T &v) const {
+        q = idx[e];
+        v = val[e];
+    }
+    // entries e0 .. e0+3, e0 a multiple of 4 (the buffers are 256-byte aligned)
+    __device__ __forceinline__ void load4(int64_t e0, int (&q)[4], T (&v)[4]) const {
+        typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+        const u16x4 w = *(const u16x4 *)(idx + e0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) q[c] = (int)(unsigned)w[c];
+        cvlhip::load4(val + e0, v);
+    }
+    __device__ __forceinline__ void store(int64_t e, int q, T v) const {
+        idx[e] = (unsigned short)q;
+        val[e] = v;
+    }
+};
+
 // One partition pass.  Region r = [r*psize, min((r+1)*psize, n)) of the
 // input is split into children of csize elements (csize divides psize,
 // psize/csize <= 256).  Block b handles tile (b % tiles_per_region) of
 // region (b / tiles_per_region).  Child c's pairs land in
 // [c*csize, c*csize + cnt[c]) of the output.
-template <class T, bool FIRST>
+template <class T, bool FIRST, class Out = Pairs<T>>
 __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pairs<T> in,
-                                                            const T *__restrict__ in_val, Pairs<T> out,
+                                                            const T *__restrict__ in_val, Out out,
                                                             int64_t n, int plog, int clog,
                                                             int64_t tiles_per_region,
                                                             const unsigned *__restrict__ prev_cnt,
@@ -358,8 +388,8 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
 
 // Block scatter: block c owns destination [c*kBlk, ...).  Its pairs are the
 // first cnt[c] entries of the same range of (idx, val).
-template <class T, bool DIRECT>
-__global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src, Pairs<T> in,
+template <class T, bool DIRECT, class In = Pairs<T>>
+__global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src, In in,
                                                                 const T *__restrict__ in_val, T *__restrict__ d,
                                                                 int64_t n, const unsigned *__restrict__ cnt,
                                                                 unsigned *__restrict__ err) {
@@ -375,7 +405,7 @@ __global__ __launch_bounds__(kPartThreads) void k_block_scatter(IndexSource src,
     int64_t count = end - base;
     if (!DIRECT && (int64_t)cnt[blk] < count) count = cnt[blk];
     auto put = [&](int64_t q, T v) {
-        const int64_t o = q - base;
+        const int64_t o = In::LOCAL ? (q & (kBlk - 1)) : q - base;
         if (o >= 0 && o < end - base) {
             s_out[o] = v;
             atomicOr(&s_mask[o >> 5], 1u << (o & 31));
@@ -437,7 +467,11 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
     auto up = up256;
     char *b = (char *)scratch;
     Pairs<T> buf[2];
+    // The last pass writes Local16 into the space of the pair buffer it
+    // would otherwise fill (2 + sizeof(T) <= the pair's bytes per element).
+    Local16<T> local[2];
     for (int k = 0; k < 2; k++) {
+        char *const b0 = b;
         if constexpr (sizeof(T) == 4) {
             buf[k].p = (uint2 *)b;
             b += up(8 * n);
@@ -447,6 +481,8 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
             buf[k].val = (T *)b;
             b += up((int64_t)sizeof(T) * n);
         }
+        local[k].idx = (unsigned short *)b0;
+        local[k].val = (T *)(b0 + up(2 * n));
     }
     const int levels = (nlog - kBlkLog + 7) / 8;
     int plog = nlog;
@@ -469,6 +505,7 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         const int64_t tpr = (rsize < n ? rsize + kPartTile - 1 : n + kPartTile - 1) / kPartTile;
         const int64_t grid = regions * tpr;
         const int out = cur == 0 ? 1 : 0;
+        const bool last = l + 1 == levels;        // children are the final blocks: write Local16
         if (cur < 0) {
             // first pass: one region, many tiles -> slots from a histogram + column scan
             unsigned *hist = (unsigned *)b;
@@ -478,19 +515,27 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
             k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
             LAUNCHCHECK();
             tile_scan(hist, grid, cnt, status);
-            k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+            if (last)
+                k_partition<T, true, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                    src, Pairs<T>{}, (const T *)s, local[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+            else
+                k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                    src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
         } else {
-            k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                src, buf[cur], nullptr, buf[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
+            if (last)
+                k_partition<T, false, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                    src, buf[cur], nullptr, local[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
+            else
+                k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                    src, buf[cur], nullptr, buf[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
         }
         LAUNCHCHECK();
         cur = out;
         plog = clog;
     }
     const int64_t blocks = ((n - 1) >> kBlkLog) + 1;
-    k_block_scatter<T, false><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(src, buf[cur], nullptr, (T *)d, n,
-                                                                              cnt, err);
+    k_block_scatter<T, false, Local16<T>><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(
+        src, local[cur], nullptr, (T *)d, n, cnt, err);
     LAUNCHCHECK();
 }
 
