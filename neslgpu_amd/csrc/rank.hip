@@ -125,15 +125,21 @@ __global__ void k_hist_scan(unsigned *hist, unsigned *starts) {
 // ---------------------------------------------------------------- one pass
 struct PassArgs {
     const void *src;         // first pass: the original values; else keys_in
-    bool vec4;               // 4-byte keys (source 0 or 2) at a 16-byte-aligned src
+    bool vec4;               // keys of 4 bytes or less (source 0, 2, 5, 6), read four at a time
     const int *idx_in;       // null on the first pass (index = position)
     void *keys_out;          // null when the next passes need no keys
+    int kout;                // bytes per key in keys_out (8, 4, 2 or 1)
+    int oshift;              // keys_out holds key >> oshift
     int *idx_out;            // the element order after this pass
     const unsigned *starts;  // 256 bucket starts of this pass
     const unsigned *tile_ofs;  // tiles x 256: elements of each digit in earlier tiles
     int64_t n;
     int shift;               // digit = (field >> shift) & 255
-    int source;              // 0: int values, 1: double values, 2: keys32, 3: keys64, 4: segment number
+    // 0: int values, 1: double values, 2: keys32, 3: keys64, 4: segment
+    // number, 5: keys16, 6: keys8.  Stored keys keep only the bits the
+    // remaining passes read, shifted down: after the second of four 8-bit
+    // passes a 32-bit key needs 16 bits, after the third 8.
+    int source;
     int up;
     SegdView seg;                  // segment of an element (segmented ranks)
     unsigned *err;                 // device error word
@@ -146,7 +152,35 @@ __device__ __forceinline__ KeyT pass_key(const PassArgs &a, int64_t e, int ix) {
     if (a.source == 1) return (KeyT)dbl_key(((const double *)a.src)[e], a.up);
     if (a.source == 2) return (KeyT)((const uint32_t *)a.src)[e];
     if (a.source == 3) return (KeyT)((const uint64_t *)a.src)[e];
+    if (a.source == 5) return (KeyT)((const uint16_t *)a.src)[e];
+    if (a.source == 6) return (KeyT)((const uint8_t *)a.src)[e];
     return (KeyT)(uint32_t)a.seg.seg_of(ix);
+}
+
+// Four consecutive stored fields from e (a multiple of 4; the arrays are
+// 256-byte aligned, the original values 16-byte aligned when vec4 is set),
+// widened to 32 bits: the raw int for source 0, else the stored key.
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 load_fields4(const PassArgs &a, int64_t e) {
+    if (a.source == 5) {
+        const u16x4 h = *(const u16x4 *)((const uint16_t *)a.src + e);
+        return u32x4{h.x, h.y, h.z, h.w};
+    }
+    if (a.source == 6) {
+        const u8x4 c = *(const u8x4 *)((const uint8_t *)a.src + e);
+        return u32x4{c.x, c.y, c.z, c.w};
+    }
+    return *(const u32x4 *)((const uint32_t *)a.src + e);
+}
+
+// keys_out[pos] = key >> oshift, in KOUT bytes.
+template <class KeyT, int KOUT>
+__device__ __forceinline__ void store_key(const PassArgs &a, int64_t pos, KeyT k) {
+    const KeyT v = k >> a.oshift;
+    if constexpr (KOUT == 1) ((uint8_t *)a.keys_out)[pos] = (uint8_t)v;
+    else if constexpr (KOUT == 2) ((uint16_t *)a.keys_out)[pos] = (uint16_t)v;
+    else ((KeyT *)a.keys_out)[pos] = v;
 }
 
 // Per-tile digit counts of one pass, as a tiles x 256 matrix (tile t =
@@ -168,7 +202,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
     for (int g = 0; g < kRankItems / 4; g++) {
         const int64_t e0 = t0 + ((int64_t)g * kRankThreads + tid) * 4;
         if (a.vec4 && e0 + 4 <= a.n) {
-            const u32x4 w = *(const u32x4 *)((const uint32_t *)a.src + e0);
+            const u32x4 w = load_fields4(a, e0);
             const uint32_t x[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
             for (int c = 0; c < 4; c++) {
@@ -193,7 +227,8 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
     for (int d = tid; d < kRadix; d += kRankThreads) hist[tile * kRadix + d] = s_hist[d];
 }
 
-template <class KeyT>
+// KOUT: bytes per key written to keys_out (a.kout), 0 when no keys are.
+template <class KeyT, int KOUT>
 __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     constexpr int W = kRankThreads / kWave;
     __shared__ unsigned s_cnt[W][kRadix];     // per-wave digit counts, then per-wave starts within a digit
@@ -227,10 +262,10 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
         if (a.vec4 && wb + kWaveSpan <= a.n) {
             static_assert(kRankItems % 4 == 0, "runs of four keys per lane");
             uint32_t *sw = (uint32_t *)s_stage + (int64_t)w * kWaveSpan;
-            auto transpose = [&](const uint32_t *src, uint32_t (&out)[kRankItems]) {
+            auto transpose = [&](auto load4, uint32_t (&out)[kRankItems]) {
 #pragma unroll
                 for (int g = 0; g < kRankItems / 4; g++) {
-                    const u32x4 v = *(const u32x4 *)(src + wb + g * 4 * kWave + 4 * lane);
+                    const u32x4 v = load4(wb + g * 4 * kWave + 4 * lane);
                     *(u32x4 *)(sw + g * 4 * kWave + 4 * lane) = v;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -243,11 +278,11 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             };
             uint32_t raw[kRankItems];
-            transpose((const uint32_t *)a.src, raw);
+            transpose([&](int64_t e) { return load_fields4(a, e); }, raw);
 #pragma unroll
             for (int j = 0; j < kRankItems; j++) key[j] = (KeyT)(a.source == 0 ? int_key((int)raw[j], a.up) : raw[j]);
             if (a.idx_in) {
-                transpose((const uint32_t *)a.idx_in, raw);
+                transpose([&](int64_t e) { return *(const u32x4 *)((const uint32_t *)a.idx_in + e); }, raw);
 #pragma unroll
                 for (int j = 0; j < kRankItems; j++) idx[j] = (int)raw[j];
             } else {
@@ -317,7 +352,9 @@ loaded:
         unsigned run = wave_inclusive_scan(t, AddU()) - t;
 #pragma unroll
         for (int i = 0; i < kRadix / kWave; i++) {
-            s_lstart[tid * (kRadix / kWave) + i] = run;
+            const int d = tid * (kRadix / kWave) + i;
+            s_lstart[d] = run;
+            s_gbase[d] -= run;       // now: global position of local slot 0 of digit d's run
             run += c[i];
         }
     }
@@ -333,16 +370,27 @@ loaded:
         if (e < a.n) {
             slot[j] = s_lstart[dig[j]] + s_cnt[w][dig[j]] + loc[j];
             s_dig[slot[j]] = (unsigned char)dig[j];
-            if (a.keys_out) s_stage[slot[j]] = key[j];
+            if constexpr (KOUT > 0) s_stage[slot[j]] = key[j];
         }
     }
     __syncthreads();
-    if (a.keys_out) {
-        for (int64_t i = tid; i < count; i += kRankThreads) {
-            const unsigned d = s_dig[i];
-            const int64_t pos = (int64_t)s_gbase[d] + (i - s_lstart[d]);
-            if (pos < a.n) ((KeyT *)a.keys_out)[pos] = s_stage[i];
-            else atomicOr(a.err, 1u);
+    // Staged element i = tid + k * kRankThreads goes to s_gbase[its digit]
+    // + i, for the keys and then for the indices.  (s_gbase[d] >= 0: the
+    // tile's elements below digit d are among all elements below it.)
+    unsigned spos[kRankItems];
+#pragma unroll
+    for (int k = 0; k < kRankItems; k++) {
+        const int i = tid + k * kRankThreads;
+        spos[k] = i < count ? s_gbase[s_dig[i]] + (unsigned)i : 0xFFFFFFFFu;
+    }
+    if constexpr (KOUT > 0) {
+#pragma unroll
+        for (int k = 0; k < kRankItems; k++) {
+            const int i = tid + k * kRankThreads;
+            if (i < count) {
+                if (spos[k] < a.n) store_key<KeyT, KOUT>(a, spos[k], s_stage[i]);
+                else atomicOr(a.err, 1u);
+            }
         }
         __syncthreads();
     }
@@ -353,11 +401,13 @@ loaded:
         if (e < a.n) s_idx[slot[j]] = idx[j];
     }
     __syncthreads();
-    for (int64_t i = tid; i < count; i += kRankThreads) {
-        const unsigned d = s_dig[i];
-        const int64_t pos = (int64_t)s_gbase[d] + (i - s_lstart[d]);
-        if (pos < a.n) a.idx_out[pos] = s_idx[i];
-        else atomicOr(a.err, 1u);
+#pragma unroll
+    for (int k = 0; k < kRankItems; k++) {
+        const int i = tid + k * kRankThreads;
+        if (i < count) {
+            if (spos[k] < a.n) a.idx_out[spos[k]] = s_idx[i];
+            else atomicOr(a.err, 1u);
+        }
     }
 }
 
@@ -451,6 +501,8 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     void *keys[2] = {b + L.keysA, b + L.keysB};
     int *idxs[2] = {(int *)(b + L.idxA), (int *)(b + L.idxB)};
     int cur = -1;   // buffer holding the current (keys, idx); -1 = original input
+    int kshift = 0;  // the stored keys are the transformed keys >> kshift
+    int kbytes = dbl ? 8 : 4;
     for (int i = 0; i < nrun; i++) {
         const int p = run[i];
         const int out = cur == 0 ? 1 : 0;
@@ -462,25 +514,49 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         a.tile_ofs = tile_ofs;
         a.idx_in = cur < 0 ? nullptr : idxs[cur];
         if (p < key_passes) {
-            a.source = cur < 0 ? (dbl ? 1 : 0) : (dbl ? 3 : 2);
+            a.source = cur < 0 ? (dbl ? 1 : 0) : (dbl ? 3 : (kbytes == 4 ? 2 : (kbytes == 2 ? 5 : 6)));
             a.src = cur < 0 ? s : keys[cur];
-            a.shift = 8 * p;
+            a.shift = 8 * p - kshift;
         } else {
             a.source = 4;
             a.src = nullptr;
             a.shift = 8 * (p - key_passes);
         }
-        a.vec4 = (a.source == 0 || a.source == 2) && ((uintptr_t)a.src & 15) == 0;
+        a.vec4 = a.source == 5 || a.source == 6 || ((a.source == 0 || a.source == 2) && ((uintptr_t)a.src & 15) == 0);
         a.keys_out = i < last_key_run ? keys[out] : nullptr;
+        a.kout = kbytes;
+        a.oshift = 0;
+        if (a.keys_out && !dbl) {
+            // the next pass that reads keys is run[i + 1] (key passes come first)
+            const int nshift = 8 * run[i + 1];
+            const int width = 32 - nshift;
+            a.kout = width > 16 ? 4 : (width > 8 ? 2 : 1);
+            a.oshift = nshift - kshift;
+        }
         a.idx_out = idxs[out];
         a.err = device_error_word();
         if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         LAUNCHCHECK();
         tile_scan(tile_ofs, tiles, nullptr, scan_status);
-        if (dbl) k_rank_pass<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
-        else k_rank_pass<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a);
+        const dim3 grid((unsigned)tiles), block(kRankThreads);
+        if (dbl) {
+            if (a.keys_out) k_rank_pass<uint64_t, 8><<<grid, block, 0, g_stream>>>(a);
+            else k_rank_pass<uint64_t, 0><<<grid, block, 0, g_stream>>>(a);
+        } else if (!a.keys_out) {
+            k_rank_pass<uint32_t, 0><<<grid, block, 0, g_stream>>>(a);
+        } else if (a.kout == 1) {
+            k_rank_pass<uint32_t, 1><<<grid, block, 0, g_stream>>>(a);
+        } else if (a.kout == 2) {
+            k_rank_pass<uint32_t, 2><<<grid, block, 0, g_stream>>>(a);
+        } else {
+            k_rank_pass<uint32_t, 4><<<grid, block, 0, g_stream>>>(a);
+        }
         LAUNCHCHECK();
+        if (a.keys_out) {
+            kshift += a.oshift;
+            kbytes = a.kout;
+        }
         cur = out;
     }
     // rank = inverse of the final order
