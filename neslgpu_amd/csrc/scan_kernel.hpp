@@ -17,12 +17,12 @@ template <> struct Vec16<double> { typedef double type __attribute__((ext_vector
 // Elements each lane holds per row: one 16-byte load's worth, by default.
 template <class Op> struct lane_values { static constexpr int value = 16 / (int)sizeof(typename Op::T); };
 
-template <class Op, int R>
+template <class Op, int R, int B = kBlock>
 struct ScanShape {
     using T = typename Op::T;
     static constexpr int V = lane_values<Op>::value;   // elements per lane per row
     static constexpr int ROW = kWave * V;            // elements per wave row
-    static constexpr int WAVES = kBlock / kWave;
+    static constexpr int WAVES = B / kWave;
     static constexpr int WAVE_ELEMS = R * ROW;
     static constexpr int TILE = WAVES * WAVE_ELEMS;
 };
@@ -167,12 +167,12 @@ template <> struct io_dual<PairIn> { static constexpr bool value = true; };
 // segmented exclusive scan (SEG=true, REDUCE=false), segmented reduce
 // (SEG=true, REDUCE=true; `out` has one element per segment) and, with a
 // dual loader (SEG=true, REDUCE=true), both at once on a pair.
-template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>>
-__global__ __launch_bounds__(kBlock) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
-                                                 uint64_t *__restrict__ status, unsigned *__restrict__ counter,
-                                                 int aligned_i, unsigned *__restrict__ err) {
+template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock>
+__global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
+                                            uint64_t *__restrict__ status, unsigned *__restrict__ counter,
+                                            int aligned_i, unsigned *__restrict__ err) {
     using T = typename Op::T;
-    using S = ScanShape<Op, R>;
+    using S = ScanShape<Op, R, B>;
     using St = TileStatus<T>;
     constexpr int V = S::V;
     constexpr uint32_t VMASK = (1u << V) - 1u;
