@@ -360,8 +360,29 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(IndexSource src, int
     const int tid = threadIdx.x;
     for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
     __syncthreads();
-    const int64_t bt = xcd_tile(blockIdx.x, gridDim.x);
+    // Rows are whole 1 KiB stores, so tiles need no XCD grouping here.
+    const int64_t bt = blockIdx.x;
     const int64_t lo = bt * kPartTile;
+    if (src.vec && lo + kPartTile <= n) {
+        // A whole tile of a plain index vector: 16-byte loads, and one
+        // unsigned compare rejects both negative and too-large indices (n
+        // >> clog <= 256, so every index below n has a digit below 256).
+        static_assert(kPartV == 4, "runs of four indices per 16-byte load");
+        u32x4 w[kPartItems / kPartV];
+#pragma unroll
+        for (int g = 0; g < kPartItems / kPartV; g++)
+            w[g] = *(const u32x4 *)(src.p + lo + ((int64_t)g * kPartThreads + tid) * kPartV);
+#pragma unroll
+        for (int g = 0; g < kPartItems / kPartV; g++) {
+            const unsigned q[4] = {w[g].x, w[g].y, w[g].z, w[g].w};
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (q[c] < (uint64_t)n) atomicAdd(&s_hist[q[c] >> clog], 1u);
+        }
+        __syncthreads();
+        for (int d = tid; d < kMaxFan; d += kPartThreads) hist[bt * kMaxFan + d] = s_hist[d];
+        return;
+    }
     int64_t qv[kPartItems];
 #pragma unroll
     for (int g = 0; g < kPartItems / kPartV; g++) {

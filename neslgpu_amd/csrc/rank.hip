@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
     const int tid = threadIdx.x;
     for (int i = tid; i < kRadix; i += kRankThreads) s_hist[i] = 0;
     __syncthreads();
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t tile = blockIdx.x;      // rows are whole 1 KiB stores: launch order
     const int64_t t0 = tile * kRankTile;
     // Item j is element t0 + ((j / 4) * kRankThreads + tid) * 4 + j % 4: runs
     // of four per lane, one 16-byte load each for 4-byte keys.
