@@ -16,7 +16,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-N = 300_007            # many 8192-key tiles and a ragged last one
+N = 300_007            # many rank tiles and a ragged last one
 
 
 @pytest.fixture(scope="module")
