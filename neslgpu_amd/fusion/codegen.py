@@ -122,9 +122,21 @@ def _reduce_op(o) -> list:
     L = [f"struct InFused{k} {{"]
     L += [f"    {f}" for f in fields]
     L += ["    template <class Op, int V>",
-          f"    __device__ __forceinline__ void load(int64_t base, int64_t n, bool, {rt} (&x)[V]) const {{"]
+          f"    __device__ __forceinline__ void load(int64_t base, int64_t n, bool aligned, {rt} (&x)[V]) const {{"]
     for j in gseg:
         L.append(f"        const long long n{j} = *(const long long *)sd{j};")
+    # Whole rows of aligned arguments: one vector load of V elements per
+    # streamed argument (`aligned` is set by the host when every streamed
+    # argument starts at a multiple of its vector size); gathered sources
+    # stay per element.
+    L += ["        if (aligned && base + V <= n) {"]
+    for i in used:
+        L.append(f"            const typename FusedVec<{CTYPE[o.params[i]]}, V>::type w{i} = "
+                 f"*(const typename FusedVec<{CTYPE[o.params[i]]}, V>::type *)(s{i} + base);")
+    L += ["#pragma unroll", "            for (int k = 0; k < V; k++) {"]
+    for i in used:
+        L.append(f"                const {CTYPE[o.params[i]]} a{i} = w{i}[k];")
+    L += [f"                x[k] = ({rt})({_expr(o, o.body)});", "            }", "            return;", "        }"]
     L += ["#pragma unroll", "        for (int k = 0; k < V; k++) {", "            const int64_t i = base + k;",
           "            if (i < n) {"]
     for i in used:
@@ -153,9 +165,12 @@ def _reduce_op(o) -> list:
           f"        const long long words = (256 + tiles * cvlhip::TileStatus<{rt}>::kWords * 8) / 4;",
           "        fused_fill<<<fused_grid(words), 256, 0, st>>>((unsigned *)scratch, words);",
           "    }",
+          "    constexpr int V = Sh::V;",
+          "    const int aligned = " + (" && ".join(
+              f"((unsigned long long)s{i} % (sizeof({CTYPE[o.params[i]]}) * V)) == 0" for i in used) or "0") + ";",
           f"    cvlhip::k_scan<Op, cvlhip::kScanRows, true, true, InFused{k}><<<(unsigned)tiles, cvlhip::kBlock, 0, st>>>(",
           f"        InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len, cvlhip::SegBits{{nullptr, nullptr, nullptr, sd}},",
-          "        status, counter, 0, (unsigned *)cvlhip_device_error_word());",
+          "        status, counter, aligned, (unsigned *)cvlhip_device_error_word());",
           f'    fused_check("fused{k}");',
           "}",
           f'extern "C" int fused{k}_scratch(int n, int m) {{',
@@ -180,6 +195,8 @@ def generate(ops, library_name: str = "fused") -> str:
             "    long long g = (n + 1023) / 1024;",
             "    return g > 16384 ? 16384 : (g < 1 ? 1 : (int)g);",
             "}", "",
+            "// V consecutive elements of one argument, read with one vector load.",
+            "template <class T, int V> struct FusedVec { typedef T type __attribute__((ext_vector_type(V))); };", "",
             "template <class T>",
             "__device__ __forceinline__ T gather(const T *src, long long n, int i) {",
             "    return i >= 0 && (long long)i < n ? src[i] : (T)0;",
