@@ -147,10 +147,15 @@ constexpr int scan_block() {
     else return SEG ? kBlock : 512;
 }
 
+// Rows per wave of a scan tile: 4-byte scans take 8 (2^28 int add scan
+// 0.502 -> 0.463 ms, segmented 0.512 -> 0.489; 16 rows in a 1024-thread
+// workgroup spill registers), double scans 16 (0.635 against 0.678 with 8).
+template <class Op, bool SEG, bool REDUCE>
+constexpr int scan_rows() { return (sizeof(typename Op::T) == 4 && !REDUCE) ? 8 : kScanRows; }
+
 template <class Op, int R, bool SEG, bool REDUCE>
 static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
     constexpr int B = scan_block<Op, SEG, REDUCE>();
-    static_assert(B >= kBlock, "scratch sizes count tiles of kBlock threads");
     using S = ScanShape<Op, R, B>;
     if (n <= 0) return;
     if (!scratch) usage_error("scan of %lld elements called without scratch", (long long)n);
@@ -167,9 +172,46 @@ static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *s
     LAUNCHCHECK();
 }
 
+// The CVL entry points: scratch_bytes counts tiles of kBlock threads and
+// kScanRows rows, so the tiles launched must not be smaller.  (The 8-row
+// schedules of the segmented-op dispatcher get twice that scratch.)
 template <class Op, bool SEG, bool REDUCE>
 static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
-    launch_scan_r<Op, kScanRows, SEG, REDUCE>(s, d, n, sb, scratch);
+    constexpr int R = scan_rows<Op, SEG, REDUCE>();
+    static_assert(scan_block<Op, SEG, REDUCE>() * R >= kBlock * kScanRows, "scratch sizes count the most tiles");
+    launch_scan_r<Op, R, SEG, REDUCE>(s, d, n, sb, scratch);
+}
+
+// Segmented reduction in two launches and no look-back: every tile reduces
+// its segments with an empty incoming prefix and records its aggregate and
+// the partial of the segment entering it (NolbRecords after the scratch
+// header); one workgroup then folds the tile aggregates in order and
+// completes those segments.  Only a segment that crosses a tile boundary
+// needs the second step, so the data is read once and no tile waits on
+// another (max_red of 2^26 doubles: 0.26 ms with a look-back).
+template <class Op, int R>
+static void launch_seg_reduce(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+    using T = typename Op::T;
+    using S = ScanShape<Op, R>;
+    if (n <= 0) return;
+    if (!scratch) usage_error("segmented reduction of %lld elements called without scratch", (long long)n);
+    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
+    if (tiles > 0x7fffffff) usage_error("vector too long");
+    void *recs = (char *)scratch + kScanHeaderBytes;
+    k_scan<Op, R, true, true, ArrayIn<T>, kBlock, true><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
+        ArrayIn<T>{(const T *)s}, (T *)d, n, sb, (uint64_t *)recs, nullptr, al16(s), device_error_word());
+    LAUNCHCHECK();
+    if (tiles > 1) {
+        k_seg_reduce_fix<Op><<<1, kFixThreads, 0, g_stream>>>(NolbRecords<T>::at(recs, tiles), tiles, (T *)d);
+        LAUNCHCHECK();
+    }
+}
+
+int64_t seg_reduce_scratch_bytes(int64_t n, int elem_bytes, int rows) {
+    const int64_t tile = (int64_t)(kBlock / kWave) * rows * kWave * (16 / elem_bytes);
+    const int64_t tiles = (n + tile - 1) / tile;
+    const int64_t recs = elem_bytes == 8 ? NolbRecords<double>::bytes(tiles) : NolbRecords<int>::bytes(tiles);
+    return kScanHeaderBytes + recs + 256;
 }
 
 template <class Op>
@@ -291,8 +333,12 @@ typename Op::T reduce_unseg(const void *s, int64_t n, void *scratch) {
     return scalar_from_bits<T>(mailbox_wait(seq));
 }
 
-template <class Op>
-void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+// Segmented reduction in tiles of R rows per wave: identities for the empty
+// segments, then the two-step reduction.  The CVL entry takes 8 rows (2^26
+// doubles in Zipf(1.5) segments: max_red 0.229 -> 0.206 ms against 16; 2^28
+// ints in geometric(256) segments: max_rez 0.575 -> 0.509 ms).
+template <class Op, int R>
+static void reduce_seg_rows(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
     using T = typename Op::T;
     if (m <= 0) return;
     if (m == 1) {                                    // vprims.c:167
@@ -304,11 +350,17 @@ void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, vo
     if (g > 4096) g = 4096;
     k_empty_identity<T><<<g, kBlock, 0, g_stream>>>((T *)d, v.lens, m, Op::cvl_id());
     LAUNCHCHECK();
-    launch_scan<Op, true, true>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+    launch_seg_reduce<Op, R>(s, d, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, scratch);
 }
 
-// The same with 8-row tiles (a schedule the segmented-op dispatcher may
-// pick; the scratch it needs is twice that of the default tiles).
+template <class Op>
+void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+    reduce_seg_rows<Op, kSegReduceRows>(d, s, sd, n, m, scratch);
+}
+
+// Other schedules the segmented-op dispatcher may pick: 8-row scan tiles
+// (the scratch they need is twice that of the default tiles) and 16-row
+// reduction tiles.
 template <class Op>
 void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
     if (m == 1) { launch_scan_r<Op, 8, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
@@ -318,15 +370,12 @@ void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, v
 
 template <class Op>
 void reduce_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    using T = typename Op::T;
-    if (m <= 0) return;
-    if (m == 1) { reduce_unseg_dev<Op>(d, s, n, scratch); return; }
-    SegdView v(sd, n, m);
-    int g = grid_for(m, kBlock);
-    if (g > 4096) g = 4096;
-    k_empty_identity<T><<<g, kBlock, 0, g_stream>>>((T *)d, v.lens, m, Op::cvl_id());
-    LAUNCHCHECK();
-    launch_scan_r<Op, 8, true, true>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+    reduce_seg_rows<Op, 8>(d, s, sd, n, m, scratch);
+}
+
+template <class Op>
+void reduce_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+    reduce_seg_rows<Op, 16>(d, s, sd, n, m, scratch);
 }
 
 // add_sed and max_red of the same values in one pass (one read of s): the
@@ -362,7 +411,7 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
     k_scan<AddMaxD, kPairRows, true, true, PairIn><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
-        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz}, status, counter, al16(s),
+        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter, al16(s),
         device_error_word());
     LAUNCHCHECK();
 }
@@ -371,7 +420,8 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);      \
     template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);    \
     template void scan_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);   \
-    template void reduce_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);
+    template void reduce_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *); \
+    template void reduce_seg_r16<OP>(void *, const void *, const void *, int64_t, int64_t, void *);
 CVLHIP_INST_SEG(AddZ)
 CVLHIP_INST_SEG(MulZ)
 CVLHIP_INST_SEG(MinZ)
@@ -418,6 +468,8 @@ using namespace cvlhip;
     int OPN##_re##TY##_scratch(int n, int m) {                                               \
         (void)m;                                                                            \
         int64_t a = scan_scratch_bytes<OP>(n), b = reduce_scratch_bytes<OP>(n);             \
+        const int64_t c = seg_reduce_scratch_bytes(n, (int)sizeof(OP::T), kSegReduceRows);  \
+        if (c > a) a = c;                                                                   \
         return units_for_bytes(a > b ? a : b);                                              \
     }                                                                                       \
     unsigned int OPN##_re##TY##_inplace(void) { return INPLACE_NONE; }

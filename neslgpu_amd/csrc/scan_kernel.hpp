@@ -92,6 +92,7 @@ struct SegBits {
     const int *wcnt;
     const int *nz;
     const void *sd;          // when set (and heads null): read the rest from this descriptor's header
+    int64_t m;               // segment count, when known (0: unknown); lets reductions see that nz is the identity
 };
 
 // Where a scan's input comes from: an array (ArrayIn), or -- in generated
@@ -163,11 +164,46 @@ struct PairIn {
 };
 template <> struct io_dual<PairIn> { static constexpr bool value = true; };
 
+// Per-tile records of a segmented reduction run without look-back (NOLB):
+// each tile reduces with an empty incoming prefix and records
+//   agg[t], flag[t]: its aggregate under the segmented operator (flag: the
+//                    tile holds a segment head; agg is then the value from
+//                    its last head to its end);
+//   slot[t], val[t]: the output slot of the segment that enters the tile
+//                    from earlier tiles and ends in it (-1 if none), and
+//                    that segment's partial value within the tile.
+// k_seg_reduce_fix then combines each such partial with the exclusive
+// prefix of the tile.  Laid out from `base` for g tiles.
+template <class T>
+struct NolbRecords {
+    T *agg;
+    int *flag;
+    int *slot;
+    T *val;
+    __host__ __device__ static int64_t up8(int64_t b) { return (b + 7) & ~(int64_t)7; }
+    __host__ __device__ static int64_t bytes(int64_t g) {
+        return up8(up8((int64_t)sizeof(T) * g) + 8 * g) + (int64_t)sizeof(T) * g;
+    }
+    __host__ __device__ static NolbRecords at(void *base, int64_t g) {
+        char *b = (char *)base;
+        NolbRecords r;
+        r.agg = (T *)b;
+        r.flag = (int *)(b + up8((int64_t)sizeof(T) * g));
+        r.slot = r.flag + g;
+        r.val = (T *)(b + up8(up8((int64_t)sizeof(T) * g) + 8 * g));
+        return r;
+    }
+};
+
 // One kernel for: unsegmented exclusive scan (SEG=false, REDUCE=false),
 // segmented exclusive scan (SEG=true, REDUCE=false), segmented reduce
 // (SEG=true, REDUCE=true; `out` has one element per segment) and, with a
 // dual loader (SEG=true, REDUCE=true), both at once on a pair.
-template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock>
+// NOLB (segmented reductions only): no look-back; `status` holds the
+// NolbRecords of gridDim.x tiles and k_seg_reduce_fix finishes the
+// segments that cross tiles.
+template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock,
+          bool NOLB = false>
 __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                             uint64_t *__restrict__ status, unsigned *__restrict__ counter,
                                             int aligned_i, unsigned *__restrict__ err) {
@@ -178,6 +214,7 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     constexpr uint32_t VMASK = (1u << V) - 1u;
     constexpr bool DUAL = io_dual<In>::value;
     static_assert(!DUAL || (SEG && REDUCE), "a dual pass is a segmented scan and reduction");
+    static_assert(!NOLB || (SEG && REDUCE && !DUAL), "only segmented reductions run without look-back");
     const Op op;
     const bool aligned = aligned_i != 0;
 
@@ -185,16 +222,21 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     __shared__ T s_wagg[S::WAVES];
     __shared__ int s_wflag[S::WAVES];
     __shared__ T s_win[S::WAVES];
+    __shared__ int s_winf[S::WAVES];   // NOLB: a head in an earlier wave of the tile
+    __shared__ int s_fix_slot;         // NOLB: this tile's record of the segment entering it
+    __shared__ T s_fix_val;
 
     // tile ids in launch order (look-back waits only on started tiles); a
     // single-tile launch passes no counter
     if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
+    if (NOLB && threadIdx.x == 0) s_fix_slot = -1;
     if constexpr (SEG) {
         if (!sb.heads) {
             const SegdView v = SegdView::from_header(sb.sd);
             sb.heads = v.heads;
             sb.wcnt = v.wcnt;
             sb.nz = v.nz;
+            sb.m = v.m;
         }
     }
     __syncthreads();
@@ -206,12 +248,27 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     T x[R][V];
     uint32_t hb[R];
     uint32_t eb[R];
+    // Reductions: the heads at or before element k of a lane's row part are
+    // wc[r] (heads before the part's bitmap word, loaded with the data and
+    // first used at the stores) + the word's heads before the part (kept in
+    // bits 16.. of eb[r]) + the part's own heads up to k.  With no empty
+    // segments the output slot of a segment is that count - 1 and nz need
+    // not be read.
+    // (A dual pass reads wc at the stores instead: its registers are at the
+    // edge of three waves per SIMD.)
+    constexpr bool WC = REDUCE && !DUAL;
+    int wc[WC ? R : 1];
+    bool nz_ident = false;
+    if constexpr (WC) {
+        if (sb.m > 0) nz_ident = sb.wcnt[(n + 31) >> 5] == sb.m;
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t base = wbase + (int64_t)r * S::ROW;
         in.template load<Op, V>(base, n, aligned, x[r]);
         hb[r] = 0;
         eb[r] = 0;
+        if constexpr (WC) wc[r] = 0;
         if constexpr (SEG) {
             if (base < n) {
                 const int64_t wd = base >> 5;
@@ -226,6 +283,10 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
                         nxt &= (last + 1 >= 32) ? 0xffffffffu : ((1u << (last + 1)) - 1u);
                     }
                     eb[r] = nxt;
+                    if constexpr (WC) {
+                        eb[r] |= (uint32_t)__popc((uint32_t)two & ((1u << sh) - 1u)) << 16;
+                        if (nxt) wc[r] = sb.wcnt[wd];
+                    }
                 }
             }
         }
@@ -294,7 +355,13 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
             tf |= f;
         }
         T P = Op::neutral();
-        if (tile == 0) {
+        if constexpr (NOLB) {
+            if (lane == 0) {
+                const NolbRecords<T> rec = NolbRecords<T>::at(status, gridDim.x);
+                rec.agg[tile] = tv;
+                rec.flag[tile] = tf;
+            }
+        } else if (tile == 0) {
             if (lane == 0) St::publish(status, 0, kPrefix, tv);
         } else {
             const bool done = SEG && tf;      // a head inside the tile: later tiles need nothing earlier
@@ -309,7 +376,10 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
         }
         if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < S::WAVES; i++) s_win[i] = wpf[i] ? wpv[i] : op(P, wpv[i]);
+            for (int i = 0; i < S::WAVES; i++) {
+                s_win[i] = wpf[i] ? wpv[i] : op(P, wpv[i]);
+                s_winf[i] = wpf[i];
+            }
         }
     }
     __syncthreads();
@@ -319,11 +389,12 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t base = wbase + (int64_t)r * S::ROW;
-        T y[V];
+        T y[V], pv[NOLB ? V : 1];
 #pragma unroll
         for (int k = 0; k < V; k++) {
             T v = x[r][k];
             if ((needs >> (r * V + k)) & 1ull) v = op(inc, v);
+            if constexpr (NOLB) pv[k] = v;
             y[k] = op(cid, v);
         }
         if constexpr (DUAL) {
@@ -344,12 +415,112 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
 #pragma unroll
             for (int k = 0; k < V; k++) {
                 if ((eb[r] >> k) & 1u) {
-                    const int64_t e = base + k;
-                    const uint32_t wd = sb.heads[e >> 5];
-                    const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);
-                    const int rk = sb.wcnt[e >> 5] + __popc(wd & mask);
-                    out[sb.nz[rk - 1]] = y[k];
+                    const int rk = wc[r] + (int)(eb[r] >> 16) + __popc(hb[r] & ((2u << k) - 1u));
+                    const int slot = nz_ident ? rk - 1 : sb.nz[rk - 1];
+                    if (NOLB && ((needs >> (r * V + k)) & 1ull) && !s_winf[w]) {
+                        // the end of the segment that entered the tile: its
+                        // value still lacks the tile's incoming prefix
+                        s_fix_slot = slot;
+                        if constexpr (NOLB) s_fix_val = pv[k];
+                    } else {
+                        out[slot] = y[k];
+                    }
                 }
+            }
+        }
+    }
+    if constexpr (NOLB) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const NolbRecords<T> rec = NolbRecords<T>::at(status, gridDim.x);
+            rec.slot[tile] = s_fix_slot;
+            rec.val[tile] = s_fix_val;
+        }
+    }
+}
+
+// Rows per wave of the tiles of a segmented reduction (the CVL entries and
+// generated fused reductions, so that both give the same bits).
+constexpr int kSegReduceRows = 8;
+
+// Second step of a NOLB segmented reduction (one workgroup of kFixThreads):
+// the exclusive prefix of every tile under the segmented operator, folded
+// in tile order, completes the segment that entered the tile:
+// out[slot[t]] = cvl_id op (prefix(t) op val[t]).
+constexpr int kFixThreads = 1024;
+template <class Op>
+__global__ __launch_bounds__(kFixThreads) void k_seg_reduce_fix(NolbRecords<typename Op::T> rec, int64_t g,
+                                                                typename Op::T *__restrict__ out) {
+    using T = typename Op::T;
+    constexpr int W = kFixThreads / kWave;
+    const Op op;
+    __shared__ T s_v[W];
+    __shared__ int s_f[W];
+    const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+    const int64_t per = (g + kFixThreads - 1) / kFixThreads;
+    const int64_t j0 = (int64_t)tid * per;
+    const int64_t j1 = j0 + per < g ? j0 + per : g;
+    // Records are read kChunk at a time, all loads in flight together: one
+    // workgroup is latency-bound, not bandwidth-bound.
+    constexpr int kChunk = 8;
+    T v = Op::neutral();
+    int f = 0;
+    for (int64_t c0 = j0; c0 < j1; c0 += kChunk) {
+        T a[kChunk];
+        int af[kChunk];
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            a[i] = c0 + i < j1 ? rec.agg[c0 + i] : Op::neutral();
+            af[i] = c0 + i < j1 ? rec.flag[c0 + i] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            if (c0 + i < j1) {
+                v = af[i] ? a[i] : op(v, a[i]);
+                f |= af[i];
+            }
+        }
+    }
+    // exclusive scan of the threads' (flag, value) pairs, in thread order
+    T iv = v;
+    int iff = f;
+    seg_wave_scan(iff, iv, op);
+    if (lane == kWave - 1) { s_v[w] = iv; s_f[w] = iff; }
+    __syncthreads();
+    if (tid == 0) {
+        T pv = Op::neutral();
+        int pf = 0;
+        for (int i = 0; i < W; i++) {
+            const T a = s_v[i];
+            const int af = s_f[i];
+            s_v[i] = pv;
+            s_f[i] = pf;
+            pv = af ? a : op(pv, a);
+            pf |= af;
+        }
+    }
+    __syncthreads();
+    T ev = shfl_up1(iv);
+    int ef = shfl_up1(iff);
+    if (lane == 0) { ev = Op::neutral(); ef = 0; }
+    T p = ef ? ev : op(s_v[w], ev);
+    const T cid = Op::cvl_id();
+    for (int64_t c0 = j0; c0 < j1; c0 += kChunk) {
+        T a[kChunk], pv[kChunk];
+        int af[kChunk], sl[kChunk];
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            const bool in = c0 + i < j1;
+            a[i] = in ? rec.agg[c0 + i] : Op::neutral();
+            af[i] = in ? rec.flag[c0 + i] : 0;
+            sl[i] = in ? rec.slot[c0 + i] : -1;
+            pv[i] = in ? rec.val[c0 + i] : Op::neutral();
+        }
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            if (c0 + i < j1) {
+                if (sl[i] >= 0) out[sl[i]] = op(cid, op(p, pv[i]));
+                p = af[i] ? a[i] : op(p, a[i]);
             }
         }
     }

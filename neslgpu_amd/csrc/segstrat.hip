@@ -170,7 +170,7 @@ static void run_arm(bool reduce, int arm, void *d, const void *s, const void *sd
     int g;
     switch (arm) {
     case 0:
-        if (reduce) reduce_seg<Op>(d, s, sd, n, m, scratch);
+        if (reduce) reduce_seg_r16<Op>(d, s, sd, n, m, scratch);
         else scan_seg<Op>(d, s, sd, n, m, scratch);
         return;
     case 1:
@@ -254,6 +254,8 @@ void cvlhip_seg_reduce(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int 
 int cvlhip_seg_scratch(int n, int m) {
     (void)m;
     int64_t a = scan_scratch_bytes<AddD>(n) * 2, b = kReduceScratchBytes;
+    const int64_t c = seg_reduce_scratch_bytes(n, 8, 8);   // the 8-row arm has the most tiles
+    if (c > a) a = c;
     return units_for_bytes(a > b ? a : b);
 }
 

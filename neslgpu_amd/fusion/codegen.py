@@ -149,7 +149,7 @@ def _reduce_op(o) -> list:
     init += [f"(const void *)s{j}" for j in gseg]
     L += [f'extern "C" void fused{k}({", ".join(hparams)}) {{',
           f"    using Op = cvlhip::{op};",
-          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kScanRows>;",
+          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kSegReduceRows>;",
           "    hipStream_t st = (hipStream_t)cvlhip_get_stream();"]
     for i in range(len(o.params)):
         if i not in used and i not in gsrc and i not in gseg and i != r:
@@ -159,25 +159,27 @@ def _reduce_op(o) -> list:
           f'    fused_check("fused{k}");',
           "    if (len <= 0) return;",
           "    const long long tiles = (len + Sh::TILE - 1) / Sh::TILE;",
-          "    unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;",
-          "    uint64_t *status = (uint64_t *)((char *)scratch + 256);",
-          "    if (tiles > 1) {",
-          f"        const long long words = (256 + tiles * cvlhip::TileStatus<{rt}>::kWords * 8) / 4;",
-          "        fused_fill<<<fused_grid(words), 256, 0, st>>>((unsigned *)scratch, words);",
-          "    }",
+          "    void *recs = (char *)scratch + 256;",
           "    constexpr int V = Sh::V;",
           "    const int aligned = " + (" && ".join(
               f"((unsigned long long)s{i} % (sizeof({CTYPE[o.params[i]]}) * V)) == 0" for i in used) or "0") + ";",
-          f"    cvlhip::k_scan<Op, cvlhip::kScanRows, true, true, InFused{k}><<<(unsigned)tiles, cvlhip::kBlock, 0, st>>>(",
+          "    // the CVL's own two-step segmented reduction (same tiles, same bits)",
+          f"    cvlhip::k_scan<Op, cvlhip::kSegReduceRows, true, true, InFused{k}, cvlhip::kBlock, true>",
+          "        <<<(unsigned)tiles, cvlhip::kBlock, 0, st>>>(",
           f"        InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len, cvlhip::SegBits{{nullptr, nullptr, nullptr, sd}},",
-          "        status, counter, aligned, (unsigned *)cvlhip_device_error_word());",
+          "        (uint64_t *)recs, nullptr, aligned, (unsigned *)cvlhip_device_error_word());",
           f'    fused_check("fused{k}");',
+          "    if (tiles > 1) {",
+          f"        cvlhip::k_seg_reduce_fix<Op><<<1, cvlhip::kFixThreads, 0, st>>>(cvlhip::NolbRecords<{rt}>::at(recs, tiles),",
+          f"                                                                       tiles, ({rt} *)d);",
+          f'        fused_check("fused{k}");',
+          "    }",
           "}",
           f'extern "C" int fused{k}_scratch(int n, int m) {{',
           "    (void)m;",
-          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kScanRows>;",
+          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kSegReduceRows>;",
           "    const long long tiles = ((long long)n + Sh::TILE - 1) / Sh::TILE;",
-          f"    return (int)((256 + tiles * cvlhip::TileStatus<{rt}>::kWords * 8 + 256 + 7) / 8);",
+          f"    return (int)((256 + cvlhip::NolbRecords<{rt}>::bytes(tiles) + 256 + 7) / 8);",
           "}",
           f'extern "C" unsigned fused{k}_inplace(void) {{ return 0u; }}', ""]
     return L
@@ -200,11 +202,6 @@ def generate(ops, library_name: str = "fused") -> str:
             "template <class T>",
             "__device__ __forceinline__ T gather(const T *src, long long n, int i) {",
             "    return i >= 0 && (long long)i < n ? src[i] : (T)0;",
-            "}", "",
-            "__global__ void fused_fill(unsigned *p, long long count) {",
-            "    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;",
-            "         i += (long long)gridDim.x * blockDim.x)",
-            "        p[i] = 0u;",
             "}", "",
             "static void fused_check(const char *what) {",
             "    hipError_t e = hipGetLastError();",
