@@ -29,7 +29,7 @@ template <> struct VecN<double> {
 };
 
 #ifndef CVLHIP_PACK_ROWS
-#define CVLHIP_PACK_ROWS 16
+#define CVLHIP_PACK_ROWS 8
 #endif
 constexpr int kPackRows = CVLHIP_PACK_ROWS;
 
@@ -151,11 +151,13 @@ int64_t pack_scratch_bytes(int64_t n) {
     return kPackHeader + 8 * tiles + 256;
 }
 
-// Threads per pack workgroup (one tile, one look-back), measured at 2^28 on
-// MI355X with 256 / 512 / 1024 threads: int 0.694 / 0.696 / 0.722 ms,
-// double (2^27) 0.625 / 0.610 / 0.575.  Scratch sizes assume 256.
+// Threads per pack workgroup (one tile, one look-back).  Measured on MI355X
+// with 8-row tiles and 256 / 512 / 1024 threads: 2^28 int 0.735 / 0.599 /
+// 0.558 ms, 2^27 double 0.462 / 0.458 / 0.459 (16-row tiles of 256
+// threads: 0.694 and 0.578).  Scratch sizes assume 256 threads, the most
+// tiles.
 template <class T>
-constexpr int pack_block() { return sizeof(T) == 4 ? kBlock : 1024; }
+constexpr int pack_block() { return 1024; }
 
 // Packs the n elements of s whose flag is set into d, which holds dlim.
 template <class T>

@@ -134,28 +134,29 @@ int64_t scan_scratch_bytes(int64_t n) {
 static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // Threads per scan workgroup.  A workgroup is one tile with one look-back,
-// so larger workgroups wait less per element; measured on MI355X (256 /
-// 512 / 1024 threads): int add scan of 2^28 0.572 / 0.556 / 0.507 ms, int
-// segmented add scan 0.526 / 0.506 / 0.570, double add scan of 2^27
-// 0.694 / 0.635 / 0.734; double segmented add scan of 2^26 in Zipf(1.5)
-// segments 0.353 / 0.390 with 16-row tiles.  Segmented reductions keep
-// 256.  Scratch sizes assume 256, the most tiles.
+// so larger workgroups wait less per element; measured on MI355X with
+// 16-row tiles (256 / 512 / 1024 threads): int add scan of 2^28 0.572 /
+// 0.556 / 0.507 ms, int segmented add scan 0.526 / 0.506 / 0.570 (the
+// 1024-thread figures spill registers; see scan_rows); with 8-row tiles,
+// double add scans of 2^27 and of 2^26 in Zipf(1.5) segments run best at
+// 1024 (0.537 and 0.317 ms against 0.678 and 0.332 at 512).  Scratch
+// sizes assume 256 threads and 16 rows, the most tiles.
 template <class Op, bool SEG, bool REDUCE>
 constexpr int scan_block() {
     if constexpr (REDUCE) return kBlock;
     else if constexpr (sizeof(typename Op::T) == 4) return SEG ? 512 : 1024;
-    else return SEG ? kBlock : 512;
+    else return 1024;
 }
 
-// Rows per wave of a scan tile: 4-byte scans take 8 (2^28 int add scan
-// 0.502 -> 0.463 ms, segmented 0.512 -> 0.489; 16 rows in a 1024-thread
-// workgroup spill registers), double scans 16 (0.635 against 0.678 with 8).
+// Rows per wave of a scan tile: 8 (16 rows in a 1024-thread workgroup
+// spill registers).  With the workgroup sizes above, against 16 rows: 2^28
+// int add scan 0.502 -> 0.463 ms, segmented 0.512 -> 0.489; 2^27 double add
+// scan 0.629 -> 0.537 ms; 2^26 doubles in Zipf(1.5) segments 0.341 -> 0.317.
 template <class Op, bool SEG, bool REDUCE>
-constexpr int scan_rows() { return (sizeof(typename Op::T) == 4 && !REDUCE) ? 8 : kScanRows; }
+constexpr int scan_rows() { return REDUCE ? kScanRows : 8; }
 
-template <class Op, int R, bool SEG, bool REDUCE>
+template <class Op, int R, bool SEG, bool REDUCE, int B = scan_block<Op, SEG, REDUCE>()>
 static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
-    constexpr int B = scan_block<Op, SEG, REDUCE>();
     using S = ScanShape<Op, R, B>;
     if (n <= 0) return;
     if (!scratch) usage_error("scan of %lld elements called without scratch", (long long)n);
@@ -358,9 +359,17 @@ void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, vo
     reduce_seg_rows<Op, kSegReduceRows>(d, s, sd, n, m, scratch);
 }
 
-// Other schedules the segmented-op dispatcher may pick: 8-row scan tiles
-// (the scratch they need is twice that of the default tiles) and 16-row
-// reduction tiles.
+// Other schedules the segmented-op dispatcher may pick: 16-row scan tiles
+// of 256 threads, 8-row scan tiles in workgroups of the default size (the
+// scratch they need is twice that of the CVL entries) and 16-row reduction
+// tiles.
+template <class Op>
+void scan_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+    if (m == 1) { launch_scan_r<Op, 16, false, false, kBlock>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
+    SegdView v(sd, n, m);
+    launch_scan_r<Op, 16, true, false, kBlock>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+}
+
 template <class Op>
 void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
     if (m == 1) { launch_scan_r<Op, 8, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
@@ -420,6 +429,7 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);      \
     template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);    \
     template void scan_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);   \
+    template void scan_seg_r16<OP>(void *, const void *, const void *, int64_t, int64_t, void *);  \
     template void reduce_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *); \
     template void reduce_seg_r16<OP>(void *, const void *, const void *, int64_t, int64_t, void *);
 CVLHIP_INST_SEG(AddZ)

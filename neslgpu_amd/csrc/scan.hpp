@@ -8,6 +8,7 @@ namespace cvlhip {
 template <class Op> void scan_unseg(void *d, const void *s, int64_t n, void *scratch);
 template <class Op> void scan_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);
 template <class Op> void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);
+template <class Op> void scan_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);
 template <class Op> void reduce_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);
 template <class Op> void reduce_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);
 template <class Op> void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch);

@@ -171,7 +171,7 @@ static void run_arm(bool reduce, int arm, void *d, const void *s, const void *sd
     switch (arm) {
     case 0:
         if (reduce) reduce_seg_r16<Op>(d, s, sd, n, m, scratch);
-        else scan_seg<Op>(d, s, sd, n, m, scratch);
+        else scan_seg_r16<Op>(d, s, sd, n, m, scratch);
         return;
     case 1:
         if (reduce) reduce_seg_r8<Op>(d, s, sd, n, m, scratch);
