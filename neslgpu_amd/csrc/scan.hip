@@ -193,13 +193,13 @@ static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scr
 template <class Op, int R>
 static void launch_seg_reduce(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
     using T = typename Op::T;
-    using S = ScanShape<Op, R>;
+    using S = ScanShape<Op, R, kSegReduceBlock>;
     if (n <= 0) return;
     if (!scratch) usage_error("segmented reduction of %lld elements called without scratch", (long long)n);
     const int64_t tiles = (n + S::TILE - 1) / S::TILE;
     if (tiles > 0x7fffffff) usage_error("vector too long");
     void *recs = (char *)scratch + kScanHeaderBytes;
-    k_scan<Op, R, true, true, ArrayIn<T>, kBlock, true><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
+    k_scan<Op, R, true, true, ArrayIn<T>, kSegReduceBlock, true><<<(unsigned)tiles, kSegReduceBlock, 0, g_stream>>>(
         ArrayIn<T>{(const T *)s}, (T *)d, n, sb, (uint64_t *)recs, nullptr, al16(s), device_error_word());
     LAUNCHCHECK();
     if (tiles > 1) {

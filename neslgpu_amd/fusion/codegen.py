@@ -149,7 +149,7 @@ def _reduce_op(o) -> list:
     init += [f"(const void *)s{j}" for j in gseg]
     L += [f'extern "C" void fused{k}({", ".join(hparams)}) {{',
           f"    using Op = cvlhip::{op};",
-          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kSegReduceRows>;",
+          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kSegReduceRows, cvlhip::kSegReduceBlock>;",
           "    hipStream_t st = (hipStream_t)cvlhip_get_stream();"]
     for i in range(len(o.params)):
         if i not in used and i not in gsrc and i not in gseg and i != r:
@@ -164,8 +164,8 @@ def _reduce_op(o) -> list:
           "    const int aligned = " + (" && ".join(
               f"((unsigned long long)s{i} % (sizeof({CTYPE[o.params[i]]}) * V)) == 0" for i in used) or "0") + ";",
           "    // the CVL's own two-step segmented reduction (same tiles, same bits)",
-          f"    cvlhip::k_scan<Op, cvlhip::kSegReduceRows, true, true, InFused{k}, cvlhip::kBlock, true>",
-          "        <<<(unsigned)tiles, cvlhip::kBlock, 0, st>>>(",
+          f"    cvlhip::k_scan<Op, cvlhip::kSegReduceRows, true, true, InFused{k}, cvlhip::kSegReduceBlock, true>",
+          "        <<<(unsigned)tiles, cvlhip::kSegReduceBlock, 0, st>>>(",
           f"        InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len, cvlhip::SegBits{{nullptr, nullptr, nullptr, sd}},",
           "        (uint64_t *)recs, nullptr, aligned, (unsigned *)cvlhip_device_error_word());",
           f'    fused_check("fused{k}");',
@@ -177,7 +177,7 @@ def _reduce_op(o) -> list:
           "}",
           f'extern "C" int fused{k}_scratch(int n, int m) {{',
           "    (void)m;",
-          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kSegReduceRows>;",
+          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kSegReduceRows, cvlhip::kSegReduceBlock>;",
           "    const long long tiles = ((long long)n + Sh::TILE - 1) / Sh::TILE;",
           f"    return (int)((256 + cvlhip::NolbRecords<{rt}>::bytes(tiles) + 256 + 7) / 8);",
           "}",
