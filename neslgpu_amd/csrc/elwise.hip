@@ -72,7 +72,9 @@ template <class T>
 __global__ void k_dist_seg(T *__restrict__ d, const T *__restrict__ v, SegdView sv) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < sv.n; i += stride) d[i] = v[sv.seg_of(i)];
+    if (i >= sv.n) return;
+    const bool ne = sv.all_nonempty();
+    for (; i < sv.n; i += stride) d[i] = v[sv.seg_of(i, ne)];
 }
 
 __global__ void k_index(int *__restrict__ d, int init, int stride_v, int64_t n) {
@@ -85,8 +87,10 @@ __global__ void k_index_seg(int *__restrict__ d, const int *__restrict__ init, c
                             SegdView sv) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (i >= sv.n) return;
+    const bool ne = sv.all_nonempty();
     for (; i < sv.n; i += stride) {
-        int g = sv.seg_of(i);
+        int g = sv.seg_of(i, ne);
         unsigned k = (unsigned)(i - sv.offs[g]);
         d[i] = (int)((unsigned)init[g] + (unsigned)st[g] * k);
     }

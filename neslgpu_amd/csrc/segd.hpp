@@ -79,6 +79,15 @@ struct SegdView {
         int r = wcnt[k >> 5] + __popc(w & mask);
         return nz[r - 1];
     }
+    // Does every segment hold an element?  Then nz is the identity.
+    __device__ __forceinline__ bool all_nonempty() const { return wcnt[(n + 31) >> 5] == m; }
+    // seg_of, skipping the nz lookup when all_nonempty() holds.
+    __device__ __forceinline__ int seg_of(int64_t k, bool nonempty) const {
+        uint32_t w = heads[k >> 5];
+        uint32_t mask = (uint32_t)(((uint64_t)2 << (k & 31)) - 1);
+        int r = wcnt[k >> 5] + __popc(w & mask);
+        return nonempty ? r - 1 : nz[r - 1];
+    }
 };
 
 }  // namespace cvlhip
