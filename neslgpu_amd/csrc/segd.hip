@@ -169,6 +169,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
     __shared__ unsigned s_tile;
     __shared__ uint64_t s_wave[kSmallThreads / kWave];
     __shared__ uint64_t s_total;
+    __shared__ int s_head[kSmallSegd];          // start of each non-empty segment of the tile, else -1
     const int tid = threadIdx.x;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
@@ -186,18 +187,51 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
     }
     uint64_t excl;
     const uint64_t P = block_tile_prefix<SegPair>(sum, s_wave, &s_total, status, tile, err, &excl);
+    int o4[kSmallPer];
 #pragma unroll
     for (int k = 0; k < kSmallPer; k++) {
         const int64_t g = g0 + k;
-        if (g < m) {
-            const uint64_t q = op(P, loc[k]);
-            const int o = (int)(uint32_t)q;
-            if (lens != lens_in) lens[g] = (int)len[k];
-            offs[g] = o;
-            if ((int)len[k] > 0) {
-                nz[q >> 32] = (int)g;
-                if (o >= 0 && (int64_t)o < n) atomicOr(heads + (o >> 5), 1u << (o & 31));
+        const uint64_t q = op(P, loc[k]);
+        o4[k] = (int)(uint32_t)q;
+        s_head[tid * kSmallPer + k] = (g < m && (int)len[k] > 0 && o4[k] >= 0 && (int64_t)o4[k] < n) ? o4[k] : -1;
+        if (g < m && (int)len[k] > 0) nz[q >> 32] = (int)g;
+    }
+    static_assert(kSmallPer == 4, "one 16-byte store per thread");
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    // (a descriptor in a CVL arena starts on an 8-byte unit: check)
+    const bool a16 = (((uintptr_t)offs | (uintptr_t)lens) & 15) == 0;
+    if (a16 && g0 + kSmallPer <= m) {    // g0 is a multiple of 4
+        *(i32x4 *)(offs + g0) = i32x4{o4[0], o4[1], o4[2], o4[3]};
+        if (lens != lens_in) *(i32x4 *)(lens + g0) = i32x4{(int)len[0], (int)len[1], (int)len[2], (int)len[3]};
+    } else {
+#pragma unroll
+        for (int k = 0; k < kSmallPer; k++) {
+            if (g0 + k < m) {
+                offs[g0 + k] = o4[k];
+                if (lens != lens_in) lens[g0 + k] = (int)len[k];
             }
+        }
+    }
+    __syncthreads();
+    // Head bits: wave w takes the tile's segments w*256 .. w*256+255, 64
+    // consecutive ones per round, ORs the bits of lanes whose heads share a
+    // bitmap word (segment starts increase with the segment index) and
+    // lets the first lane of each such run set them with one atomic.
+    {
+        const int w = tid / kWave, lane = tid % kWave;
+#pragma unroll
+        for (int k = 0; k < kSmallPer; k++) {
+            const int o = s_head[w * kWave * kSmallPer + k * kWave + lane];
+            int key = o >= 0 ? (o >> 5) : -1 - lane;       // an empty segment matches no neighbour
+            uint32_t v = o >= 0 ? 1u << (o & 31) : 0u;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const uint32_t vo = __shfl_down(v, d);
+                const int ko = __shfl_down(key, d);
+                if (lane + d < kWave && ko == key) v |= vo;
+            }
+            const int kp = __shfl_up(key, 1);
+            if (o >= 0 && (lane == 0 || kp != key)) atomicOr(heads + (o >> 5), v);
         }
     }
     if (tile == (int64_t)gridDim.x - 1 && tid == 0) {
