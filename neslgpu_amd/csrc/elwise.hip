@@ -74,7 +74,19 @@ __global__ void k_dist_seg(T *__restrict__ d, const T *__restrict__ v, SegdView 
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     if (i >= sv.n) return;
     const bool ne = sv.all_nonempty();
-    for (; i < sv.n; i += stride) d[i] = v[sv.seg_of(i, ne)];
+    // four elements per step, their lookups in flight together: each is a
+    // chain of two dependent loads (bitmap word, then the segment's value)
+    for (; i < sv.n; i += 4 * stride) {
+        int g[4];
+        T x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) g[u] = i + u * stride < sv.n ? sv.seg_of(i + u * stride, ne) : 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) x[u] = i + u * stride < sv.n ? v[g[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (i + u * stride < sv.n) d[i + u * stride] = x[u];
+    }
 }
 
 __global__ void k_index(int *__restrict__ d, int init, int stride_v, int64_t n) {
@@ -89,10 +101,18 @@ __global__ void k_index_seg(int *__restrict__ d, const int *__restrict__ init, c
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     if (i >= sv.n) return;
     const bool ne = sv.all_nonempty();
-    for (; i < sv.n; i += stride) {
-        int g = sv.seg_of(i, ne);
-        unsigned k = (unsigned)(i - sv.offs[g]);
-        d[i] = (int)((unsigned)init[g] + (unsigned)st[g] * k);
+    for (; i < sv.n; i += 4 * stride) {        // four elements per step, as in k_dist_seg
+        int g[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) g[u] = i + u * stride < sv.n ? sv.seg_of(i + u * stride, ne) : 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t e = i + u * stride;
+            if (e < sv.n) {
+                const unsigned k = (unsigned)(e - sv.offs[g[u]]);
+                d[e] = (int)((unsigned)init[g[u]] + (unsigned)st[g[u]] * k);
+            }
+        }
     }
 }
 

@@ -403,9 +403,14 @@ int64_t add_max_pair_scratch_bytes(int64_t n) {
     return kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8 + 256;
 }
 
+#ifndef CVLHIP_PAIR_BLOCK
+#define CVLHIP_PAIR_BLOCK 256
+#endif
+constexpr int kPairBlock = CVLHIP_PAIR_BLOCK;
+
 void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd, int64_t n, int64_t m,
                   void *scratch) {
-    using S = ScanShape<AddMaxD, kPairRows>;
+    using S = ScanShape<AddMaxD, kPairRows, kPairBlock>;
     if (m <= 0) return;
     if (!scratch) usage_error("add_sed/max_red pair called without scratch");
     SegdView v(sd, n, m);
@@ -419,7 +424,7 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8) / 4);
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
-    k_scan<AddMaxD, kPairRows, true, true, PairIn><<<(unsigned)tiles, kBlock, 0, g_stream>>>(
+    k_scan<AddMaxD, kPairRows, true, true, PairIn, kPairBlock><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
         PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter, al16(s),
         device_error_word());
     LAUNCHCHECK();
