@@ -78,11 +78,12 @@ inline int64_t tile_scan_status_words(int64_t tiles) {
 }
 
 // Scan m (tiles x 256) in place on the library stream; `status` holds
-// tile_scan_status_words(tiles) words and is cleared here.
-inline void tile_scan(unsigned *m, int64_t tiles, unsigned *totals, uint64_t *status) {
+// tile_scan_status_words(tiles) words and is cleared here unless the caller
+// cleared it already (`cleared`).
+inline void tile_scan(unsigned *m, int64_t tiles, unsigned *totals, uint64_t *status, bool cleared = false) {
     if (tiles <= 0) return;
     const int64_t groups = (tiles + kTileScanRows - 1) / kTileScanRows;
-    fill_u32(status, 0, 2 * kTileScanCols * groups);
+    if (!cleared) fill_u32(status, 0, 2 * kTileScanCols * groups);
     k_tile_scan<<<(unsigned)groups, kTileScanCols, 0, g_stream>>>(m, tiles, totals, status, device_error_word());
     LAUNCHCHECK();
 }

@@ -506,6 +506,27 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         local[k].val = (T *)(b0 + up(2 * n));
     }
     const int levels = (nlog - kBlkLog + 7) / 8;
+    // Per level: its child counters (>= 256 entries: the column scan writes
+    // 256 totals); then the first pass's column-scan status and histogram.
+    // The counters and the status are contiguous and cleared in one launch.
+    unsigned *cnts[4];
+    {
+        int pl = nlog;
+        for (int l = 0; l < levels; l++) {
+            const int bits = l + 1 < levels ? 8 : pl - kBlkLog;
+            const int cl = pl - bits;
+            const int64_t nchild = ((n - 1) >> cl) + 1;
+            cnts[l] = (unsigned *)b;
+            b += up(4 * ((nchild > kMaxFan ? nchild : kMaxFan) + 1));
+            pl = cl;
+        }
+    }
+    const int64_t grid0 = (n + kPartTile - 1) / kPartTile;    // tiles of the first pass (one region)
+    uint64_t *status0 = (uint64_t *)b;
+    b += up(8 * tile_scan_status_words(grid0));
+    fill_u32(cnts[0], 0, (int64_t)((char *)b - (char *)cnts[0]) / 4);
+    unsigned *hist0 = (unsigned *)b;
+    b += up(4 * (int64_t)kMaxFan * grid0);
     int plog = nlog;
     int cur = -1;
     unsigned *cnt = nullptr, *prev = nullptr;
@@ -515,12 +536,8 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         // per (tile, child), so the narrower fan belongs there.
         const int bits = l + 1 < levels ? 8 : plog - kBlkLog;
         const int clog = plog - bits;
-        const int64_t nchild = ((n - 1) >> clog) + 1;
         prev = cnt;
-        cnt = (unsigned *)b;                       // >= 256 entries: the column scan writes 256 totals
-        const int64_t ncnt = (nchild > kMaxFan ? nchild : kMaxFan) + 1;
-        b += up(4 * ncnt);
-        fill_u32(cnt, 0, ncnt);
+        cnt = cnts[l];
         const int64_t regions = ((n - 1) >> plog) + 1;
         const int64_t rsize = (int64_t)1 << plog;
         const int64_t tpr = (rsize < n ? rsize + kPartTile - 1 : n + kPartTile - 1) / kPartTile;
@@ -529,13 +546,12 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         const bool last = l + 1 == levels;        // children are the final blocks: write Local16
         if (cur < 0) {
             // first pass: one region, many tiles -> slots from a histogram + column scan
-            unsigned *hist = (unsigned *)b;
-            b += up(4 * (int64_t)kMaxFan * grid);
-            uint64_t *status = (uint64_t *)b;
-            b += up(8 * tile_scan_status_words(grid));
+            if (grid != grid0) usage_error("permute: first-pass tiling");
+            unsigned *hist = hist0;
+            uint64_t *status = status0;
             k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
             LAUNCHCHECK();
-            tile_scan(hist, grid, cnt, status);
+            tile_scan(hist, grid, cnt, status, true);
             if (last)
                 k_partition<T, true, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
                     src, Pairs<T>{}, (const T *)s, local[out], n, plog, clog, tpr, nullptr, cnt, hist, err);

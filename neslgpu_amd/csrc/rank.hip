@@ -433,7 +433,8 @@ struct RankScratch {
         counters = up(starts + 4 * 12 * kRadix);
         status = up(counters + 4 * 16);
         scan_status = up(status + 4 * kRadix * (tiles > 0 ? tiles : 1));
-        invert = up(scan_status + 8 * tile_scan_status_words(tiles > 0 ? tiles : 1));
+        // one column-scan status region per pass (at most 12), cleared together
+        invert = up(scan_status + 12 * 8 * tile_scan_status_words(tiles > 0 ? tiles : 1));
         bytes = up(invert + partition_permute_scratch_bytes(n, 4));
     }
 };
@@ -498,6 +499,8 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     int last_key_run = -1;
     for (int i = 0; i < nrun; i++) if (run[i] < key_passes) last_key_run = i;
 
+    const int64_t sw = tile_scan_status_words(tiles);
+    fill_u32(scan_status, 0, 2 * sw * nrun);       // the status regions of every pass that runs
     void *keys[2] = {b + L.keysA, b + L.keysB};
     int *idxs[2] = {(int *)(b + L.idxA), (int *)(b + L.idxB)};
     int cur = -1;   // buffer holding the current (keys, idx); -1 = original input
@@ -538,7 +541,7 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
         LAUNCHCHECK();
-        tile_scan(tile_ofs, tiles, nullptr, scan_status);
+        tile_scan(tile_ofs, tiles, nullptr, scan_status + i * sw, true);
         const dim3 grid((unsigned)tiles), block(kRankThreads);
         if (dbl) {
             if (a.keys_out) k_rank_pass<uint64_t, 8><<<grid, block, 0, g_stream>>>(a);

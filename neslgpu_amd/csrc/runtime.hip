@@ -108,6 +108,26 @@ void fill_u32(void *p, uint32_t v, int64_t count) {
     LAUNCHCHECK();
 }
 
+// Two regions cleared in one launch (element i < n1 of the first, else
+// element i - n1 of the second): a launch saved where both are small.
+__global__ void k_zero2_u32(uint32_t *__restrict__ p1, int64_t n1, uint32_t *__restrict__ p2, int64_t n2) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2; i += stride) {
+        if (i < n1) p1[i] = 0u;
+        else p2[i - n1] = 0u;
+    }
+}
+
+void zero2_u32(void *p1, int64_t n1, void *p2, int64_t n2) {
+    if (n1 < 0) n1 = 0;
+    if (n2 < 0) n2 = 0;
+    if (n1 + n2 <= 0) return;
+    int g = grid_for(n1 + n2, kBlock * 4);
+    if (g > 8192) g = 8192;
+    k_zero2_u32<<<g, kBlock, 0, g_stream>>>((uint32_t *)p1, n1, (uint32_t *)p2, n2);
+    LAUNCHCHECK();
+}
+
 // Overlap-safe move of `bytes` bytes (ref facilt.c:161-183 walks forward
 // when dest < src and backward otherwise).  Each block copies a chunk that
 // does not overlap the chunks still to be read by later blocks in the same

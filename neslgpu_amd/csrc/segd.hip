@@ -309,8 +309,7 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     unsigned *counters = (unsigned *)scratch;                          // [0] pass 1, [1] pass 2
     uint64_t *st_a = (uint64_t *)((char *)scratch + 256);
     uint64_t *st_b = st_a + 2 * (ta + 1);
-    fill_u32(scratch, 0, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4);
-    fill_u32(heads, 0, L.words + 8);
+    zero2_u32(scratch, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4, heads, L.words + 8);
     if (m > 0) {
         k_segd_offsets<<<(unsigned)ta, kSmallThreads, 0, g_stream>>>((const int *)lens_in, hdr, lens, offs, nz, heads,
                                                                      n, m, st_a, counters, device_error_word());
