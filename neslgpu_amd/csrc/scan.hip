@@ -148,12 +148,15 @@ constexpr int scan_block() {
     else return 1024;
 }
 
-// Rows per wave of a scan tile: 8 (16 rows in a 1024-thread workgroup
-// spill registers).  With the workgroup sizes above, against 16 rows: 2^28
-// int add scan 0.502 -> 0.463 ms, segmented 0.512 -> 0.489; 2^27 double add
-// scan 0.629 -> 0.537 ms; 2^26 doubles in Zipf(1.5) segments 0.341 -> 0.317.
+// Rows per wave of a scan tile (16 rows in a 1024-thread workgroup spill
+// registers).  With the workgroup sizes above, 8 rows against 16: 2^28 int
+// add scan 0.502 -> 0.463 ms, segmented 0.512 -> 0.489; 2^27 double add
+// scan 0.629 -> 0.537 ms; 2^26 doubles in Zipf(1.5) segments 0.341 ->
+// 0.317.  12 rows (128 registers, no spill) take the plain int scan to
+// 0.443 ms but slow the others (segmented int 0.485 -> 0.543, double
+// 0.536 -> 0.652), so only plain 4-byte scans use them.
 template <class Op, bool SEG, bool REDUCE>
-constexpr int scan_rows() { return REDUCE ? kScanRows : 8; }
+constexpr int scan_rows() { return REDUCE ? kScanRows : ((sizeof(typename Op::T) == 4 && !SEG) ? 12 : 8); }
 
 template <class Op, int R, bool SEG, bool REDUCE, int B = scan_block<Op, SEG, REDUCE>()>
 static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
