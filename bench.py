@@ -475,9 +475,10 @@ def main():
     dom = max(prims, key=lambda k: prims[k]["ms"])
     kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce (MaxZ)",
                  "add_sez": "k_scan (AddZ, segmented)", "pk2_luz": "k_pack<int>",
-                 "smp_puz": "k_part_hist + k_partition<int> x2 + k_block_scatter<int>",
-                 "rku_luz": "k_key_hist + 4 x (k_rank_hist + k_col_scan + k_rank_pass<uint32_t>) + "
-                            "inversion (k_part_hist + k_partition<int> x2 + k_block_scatter<int>)",
+                 "smp_puz": "k_part_hist + k_tile_scan + k_partition<int> x2 (pairs, then 16-bit offsets) "
+                            "+ k_block_scatter<int>",
+                 "rku_luz": "k_key_hist + 4 x (k_rank_hist + k_tile_scan + k_rank_pass) + "
+                            "inversion (k_part_hist + k_tile_scan + k_partition<int> x2 + k_block_scatter<int>)",
                  "cpy_wuz": "__amd_rocclr_copyBuffer"}
     roof = {
         "bound": "hbm",
