@@ -473,7 +473,7 @@ def main():
         prims[k] = {"ms": round(t, 4), "GB/s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                     "bytes": byts[k]}
     dom = max(prims, key=lambda k: prims[k]["ms"])
-    kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce (MaxZ)",
+    kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce_strided + k_reduce_finish (MaxZ)",
                  "add_sez": "k_scan (AddZ, segmented)", "pk2_luz": "k_pack<int>",
                  "smp_puz": "k_part_hist + k_tile_scan + k_partition<int> x2 (pairs, then 16-bit offsets) "
                             "+ k_block_scatter<int>",
