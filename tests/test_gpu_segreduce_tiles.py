@@ -1,0 +1,81 @@
+"""Segmented reductions at tile boundaries.
+
+The segmented reductions run in two steps without a look-back: each tile
+reduces the segments that start and end in it and records the partial of
+the one that enters it from earlier tiles; a second launch completes those
+from the tiles' aggregates.  The cases below put segment boundaries exactly
+on, just before and just after tile boundaries, make one segment span many
+tiles, put empty segments at tile boundaries and pack thousands of
+one-element segments around them.  Results must equal the C oracle's
+(nesl/cvl/serial/vprims.c:128-201) exactly for integers and min/max, and be
+within the compensated-sum bound for double +, through the CVL entry
+points (8-row tiles) and both tile schedules of cvlhip_seg_reduce."""
+import numpy as np
+import pytest
+
+from gpu_util import eq, fp_add_tol_vec, host
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# elements per tile of a segmented reduction: 512 threads x rows x 16 bytes
+TILE8 = {"z": 8 * 8 * 64 * 4, "d": 8 * 8 * 64 * 2}
+
+
+@pytest.fixture(scope="module")
+def cv():
+    from neslgpu_amd.cvl import Cvl
+    return Cvl()
+
+
+def patterns(T, rng):
+    ragged = rng.integers(0, 3 * T, 40)
+    return {
+        "aligned": [T, T, T, T],
+        "spanning": [3, 5 * T + 1, T + 1],
+        "near": [T - 1, 2, T - 1, 1, T + 1, T - 2, 3, 2 * T],
+        "empties": [T, 0, T, 0, 0, T - 3, 3, 0, T, 0],
+        "singletons": [1] * (T + 20) + [T] + [1] * 40,
+        "ragged": ragged.tolist(),
+    }
+
+
+def values(ty, n, rng):
+    if ty == "d":
+        return rng.uniform(-1, 1, n)
+    return rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
+
+
+@pytest.mark.parametrize("ty", ["z", "d"])
+def test_segmented_reductions_at_tile_boundaries(cv, ty):
+    import torch
+    from neslgpu_amd import dispatch as D
+    rng = np.random.default_rng(77 if ty == "z" else 78)
+    L = cv.lib
+    for name, lens in patterns(TILE8[ty], rng).items():
+        lens = np.asarray(lens, np.int32)
+        n, m = int(lens.sum()), lens.size
+        x = values(ty, n, rng)
+        sd = cv.segd(lens)
+        dx = torch.from_numpy(x).to(cv.device)
+        out = torch.empty(m, dtype=dx.dtype, device=cv.device)
+        for op in ("add", "max", "min") + (("mul",) if ty == "z" else ()):
+            if ty == "d" and op == "add":
+                want, absum = O.segreduce_add_neumaier(x, lens)
+            else:
+                want = O.segreduce(op, ty, x, lens)
+            runs = [("entry", lambda: getattr(L, f"{op}_re{ty}")(
+                out, dx, sd, n, m, cv.scratch(getattr(L, f"{op}_re{ty}_scratch")(n, m))))]
+            for arm in (0, 1):       # 16-row and 8-row tiles
+                runs.append((f"arm{arm}", lambda arm=arm: L.cvlhip_seg_reduce(
+                    D.OPS[op], 1 if ty == "d" else 0, arm, out, dx, sd, n, m,
+                    cv.scratch(L.cvlhip_seg_scratch(n, m)))))
+            for how, run in runs:
+                print(name, op, ty, how, n, m, flush=True)
+                run()
+                got = host(out)
+                if ty == "d" and op == "add":
+                    assert np.all(np.abs(got - want) <= fp_add_tol_vec(absum, lens)), (name, op, how)
+                else:
+                    eq(got, want, f"{name} {op}_re{ty} {how}")
+    assert L.cvlhip_check_device_errors() == 0
