@@ -62,6 +62,8 @@ def shapes():
         ("large_long", split(1 << 21, 70_000)),
         ("large_many", split(300_000, 200_000, 0.3)),
         ("all_empty", np.zeros(5000, np.int32)),
+        ("singletons", np.ones(100_003, np.int32)),     # every bit of every bitmap word set
+        ("pairs_empty", np.tile(np.array([1, 0, 2], np.int32), 40_000)),
         ("empty_ends", np.concatenate([[0, 0], split(200_000, 9000), [0]]).astype(np.int32)),
     ]
 
