@@ -229,7 +229,10 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     // tile ids in launch order (look-back waits only on started tiles); a
     // single-tile launch passes no counter
     if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
-    if (NOLB && threadIdx.x == 0) s_fix_slot = -1;
+    if (NOLB && threadIdx.x == 0) {
+        s_fix_slot = -1;
+        s_fix_val = Op::neutral();
+    }
     if constexpr (SEG) {
         if (!sb.heads) {
             const SegdView v = SegdView::from_header(sb.sd);
