@@ -249,7 +249,9 @@ def main(argv=None):
     n_eval = 1 << 26
     t0 = time.time()
     X, T, M, meta = gather(cv, TRAIN_WORKLOADS, n_train)
-    X2, T2, M2, meta2 = gather(cv, TRAIN_WORKLOADS[::3], n_eval, seed=5)
+    # a subset again at the evaluation size, with the steep Zipf tails whose
+    # very short segments favour other tiles than the rest
+    X2, T2, M2, meta2 = gather(cv, TRAIN_WORKLOADS[::3] + [("zipf", 2.2), ("zipf", 3.0)], n_eval, seed=5)
     Xtr = np.concatenate([X, X2]); Ttr = np.concatenate([T, T2]); Mtr = np.concatenate([M, M2])
     disp = Dispatcher(cv)
     losses = disp.train(Xtr, Ttr, Mtr, epochs=3000)
