@@ -56,18 +56,7 @@ def parse():
     return p.parse_args()
 
 
-def geometric_lengths(n: int, mean: int, seed: int) -> np.ndarray:
-    """Segment lengths >= 1 with a geometric distribution, summing to n."""
-    rng = np.random.default_rng(seed)
-    m_guess = n // mean + 1024
-    lens = rng.geometric(1.0 / mean, size=m_guess).astype(np.int64)
-    cs = np.cumsum(lens)
-    k = int(np.searchsorted(cs, n))
-    lens = lens[: k + 1]
-    lens[-1] -= int(cs[k] - n)
-    lens = lens[lens > 0]
-    assert int(lens.sum()) == n
-    return lens.astype(np.int32)
+from neslgpu_amd.workloads import geometric_lengths  # noqa: E402
 
 
 class Sweep:
@@ -129,6 +118,7 @@ class Sweep:
             "pk2_luz": 8 * n + 4 * self.kept,
             "smp_puz": 12 * n,
             "rku_luz": 8 * nr,
+            "pack_full": 12 * n + 4 * self.kept,
             "cpy_wuz": 8 * n,
         }
 
@@ -146,6 +136,9 @@ class Sweep:
             L.smp_puz(self.d, self.s, self.perm, n, self.scr)
         elif name == "rku_luz":
             L.rku_luz(self.rank, self.keys, self.nr, self.scr)
+        elif name == "pack_full":          # VCODE PACK: count (host read), then compact
+            kept = L.pk1_luv(self.flags, n, self.scr)
+            L.pk2_luz(self.d, self.s, self.flags, n, kept, self.scr)
         elif name == "cpy_wuz":            # calibration copy (hipMemcpy device to device)
             L.cpy_wuz(self.d, self.s, n, self.scr)
 
@@ -172,51 +165,61 @@ def time_gpu(fn, reps: int, torch) -> float:
 def irregular(log2n: int, reps: int) -> dict:
     """Segmented float64 +-scan and max-reduce over Zipf(1.5) segment lengths
     clipped to [1, 2^20] (2^26 values uniform in [-1, 1]): every schedule
-    timed (median of `reps`, HIP events), the learned dispatcher's pick, the
-    exhaustive autotuner's best, the CVL entry points add_sed / max_red, and
-    both operations in one pass (cvlhip_add_sed_max_red)."""
+    timed (median of `reps`, HIP events), the exhaustive autotuner's best,
+    and the learned dispatch as a program gets it: the CVL entry points
+    add_sed / max_red on a descriptor whose schedules mke_fov picked with the
+    library's network, plus the cost of that decision (mke_fov with the
+    dispatcher on minus off).  Also both operations in one pass
+    (cvlhip_add_sed_max_red)."""
     import torch
     from neslgpu_amd import dispatch as D
-    from neslgpu_amd.cvl import Cvl
+    from neslgpu_amd.cvl import Cvl, Segd
     cv = Cvl()
     L = cv.lib
     n = 1 << log2n
     x = torch.empty(n, dtype=torch.float64, device=cv.device)
     L.cvlhip_fill_splitmix_f64(x, n, SEED + 3, -1.0, 1.0)
     lens = D.lengths("zipf", n, 1.5, np.random.default_rng(SEED + 3 & 0x7FFFFFFF), lmax=1 << 20)
-    sd = cv.segd(lens)
     m = int(lens.size)
+    lens_dev = torch.from_numpy(lens).to(cv.device)
+    units = L.siz_fos(n, m)
+    bufs = {on: torch.zeros(units * 2 + 64, dtype=torch.int32, device=cv.device) for on in (0, 1)}
+    bscr = torch.empty(L.mke_fov_scratch(n, m) * 8 + 256, dtype=torch.uint8, device=cv.device)
+    build = {}
+    for on in (0, 1):
+        L.cvlhip_sched_enable(on)
+        build[on] = float(np.median([time_gpu(lambda on=on: L.mke_fov(bufs[on], lens_dev, n, m, bscr), 1, torch)
+                                     for _ in range(reps)]))
+    L.cvlhip_sched_enable(1)
+    decision = max(0.0, build[1] - build[0])
+    sd = Segd(bufs[1], n, m)
     d = torch.empty_like(x)
     r = torch.empty(m, dtype=torch.float64, device=cv.device)
     tuner = D.Autotuner(cv, reps=reps)
-    disp = None
-    if os.path.exists(D.MODEL_PATH):
-        disp = D.Dispatcher(cv).load()
     scr = cv.scratch(max(L.add_sed_scratch(n, m), L.max_red_scratch(n, m), L.cvlhip_seg_scratch(n, m),
                          L.cvlhip_add_sed_max_red_scratch(n, m)))
     out = {"workload": f"2^{log2n} float64 uniform [-1,1], Zipf(1.5) segment lengths clipped to [1,2^20]",
-           "n": n, "segments": m, "arms": D.ARMS}
-    tot = {"learned": 0.0, "autotuned": 0.0, "cvl_entry": 0.0}
+           "n": n, "segments": m, "arms": D.ARMS, "model_built_in": bool(L.cvlhip_sched_has_model()),
+           "mke_fov_ms": {"dispatcher_off": round(build[0], 4), "dispatcher_on": round(build[1], 4)},
+           "decision_ms": round(decision, 4)}
+    tot = {"learned": decision, "autotuned": 0.0}
     fixed = np.zeros(len(D.ARMS))
     for kind, op, dst, byts in (("scan", "add", d, 16 * n + 4 * m), ("reduce", "max", r, 8 * n + 12 * m)):
         t = tuner.tune(kind, op, "d", dst, x, sd)
         fixed += t
         best = int(np.argmin(t))
-        pick = disp.choose(D.features(cv, sd, kind, "d")) if disp is not None else None
+        pick = L.cvlhip_segd_pick(sd, 1 if kind == "reduce" else 0, 8)
         if kind == "scan":
             entry = lambda: L.add_sed(d, x, sd, n, m, scr)
         else:
             entry = lambda: L.max_red(r, x, sd, n, m, scr)
         te = float(np.median([time_gpu(entry, 1, torch) for _ in range(reps)]))
         rec = {"ms_per_arm": [round(float(v), 4) for v in t], "autotuned_arm": D.ARMS[best],
-               "autotuned_ms": round(float(t[best]), 4), "cvl_entry_ms": round(te, 4),
-               "compulsory_bytes": byts, "autotuned_GB/s": round(byts / t[best] / 1e6, 1)}
+               "autotuned_ms": round(float(t[best]), 4), "learned_arm": D.ARMS[pick] if pick >= 0 else None,
+               "learned_ms": round(te, 4), "compulsory_bytes": byts,
+               "learned_GB/s": round(byts / te / 1e6, 1)}
         tot["autotuned"] += float(t[best])
-        tot["cvl_entry"] += te
-        if pick is not None:
-            rec["learned_arm"] = D.ARMS[pick]
-            rec["learned_ms"] = round(float(t[pick]), 4)
-            tot["learned"] += float(t[pick])
+        tot["learned"] += te
         out[f"{op}_{kind}"] = rec
     # both operations in one pass over the values (cvlhip_add_sed_max_red)
     pair = lambda: L.cvlhip_add_sed_max_red(d, r, x, sd, n, m, scr)
@@ -226,11 +229,8 @@ def irregular(log2n: int, reps: int) -> dict:
                             "GB/s": round((16 * n + 12 * m) / tp / 1e6, 1)}
     out["total_ms"] = {k: round(v, 4) for k, v in tot.items()}
     out["fixed_arm_total_ms"] = {D.ARMS[a]: round(float(fixed[a]), 4) for a in range(len(D.ARMS))}
-    if disp is not None:
-        out["learned_regret_vs_autotuned"] = round(tot["learned"] / tot["autotuned"] - 1.0, 4)
-        out["learned_vs_best_fixed"] = round(tot["learned"] / float(fixed.min()) - 1.0, 4)
-    else:
-        out["learned"] = "no trained model (neslgpu_amd/dispatch_model.npz missing)"
+    out["learned_regret_vs_autotuned"] = round(tot["learned"] / tot["autotuned"] - 1.0, 4)
+    out["learned_vs_best_fixed"] = round(tot["learned"] / float(fixed.min()) - 1.0, 4)
     return out
 
 
@@ -239,49 +239,20 @@ PROGRAMS = os.path.join(ROOT, "tests", "golden", "vcode", "programs")
 OWN_PROGRAMS = os.path.join(ROOT, "neslgpu_amd", "vcode", "programs")
 
 
-def power_law_rows(m: int, nnz: int, mean: float, lmax: int, seed: int) -> np.ndarray:
-    """m row lengths in [0, lmax] from a discrete power law with the given
-    mean, adjusted by +-1 steps so they sum to exactly nnz."""
-    L = np.arange(1, lmax + 1, dtype=np.float64)
-    lo, hi = 1.01, 4.0
-    for _ in range(60):                       # exponent giving the mean
-        a = 0.5 * (lo + hi)
-        p = L ** -a
-        if (p * L).sum() / p.sum() > mean:
-            lo = a
-        else:
-            hi = a
-    p = L ** -a
-    rng = np.random.default_rng(seed)
-    rows = rng.choice(np.arange(1, lmax + 1), size=m, p=p / p.sum()).astype(np.int64)
-    diff = nnz - int(rows.sum())
-    while diff != 0:
-        k = min(abs(diff), m)
-        idx = rng.choice(m, size=k, replace=False)
-        if diff > 0:
-            idx = idx[rows[idx] < lmax]
-            rows[idx] += 1
-            diff -= idx.size
-        else:
-            idx = idx[rows[idx] > 0]
-            rows[idx] -= 1
-            diff += idx.size
-    return rows.astype(np.int32)
-
-
 def program_workloads():
     """(name, program text, stdin, fused library or None, graphs, repeats,
     description) of the whole-program workloads."""
     from neslgpu_amd import vinterp as V
-    from neslgpu_amd.build import FUSED_DIR
+    from neslgpu_amd.build import FUSED_DIR, generated
     qs = V.set_function_int_const(V.set_size(V.read(os.path.join(PROGRAMS, "qs.1000.vcode")), 1 << 20),
                                   "QS_4", 100, 2147483647)
     qh = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.vcode")), 1 << 24)
     qhf = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.fcode")), 1 << 24)
-    rows = power_law_rows(1 << 22, 1 << 26, 16.0, 1 << 16, SEED + 4 & 0x7FFFFFFF)
-    spmv_in = "( " + " ".join(map(str, rows.tolist())) + " )\n( " + str(1 << 22) + " )\n"
+    from neslgpu_amd import workloads as W
+    rows = W.spmv_rows()
+    spmv_in = W.spmv_stdin(rows, W.SPMV_ROWS)
     spmv = V.read(os.path.join(OWN_PROGRAMS, "spmv.vcode"))
-    spmvf = V.read(os.path.join(OWN_PROGRAMS, "spmv.fcode"))
+    spmvf = V.read(generated("spmv", "fcode"))
     return [
         ("quicksort", qs, None, None, False, 1, "qs.nesl: 2^20 keys uniform in [0, 2^31-1) (RAND), sorted"),
         ("quickhull", qh, None, None, False, 1, "quickhull.nesl: 2^24 points uniform in the unit disk"),
@@ -289,9 +260,9 @@ def program_workloads():
          "same, FUSED super-ops (generated HIP)"),
         ("spmv", spmv, spmv_in, None, False, 10, "y = A x, CSR 2^22 rows, power-law lengths (mean 16, "
          "max 2^16), 2^26 nnz, float64; BPERMUTE, *, +_REDUCE"),
-        ("spmv_fused", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), False, 10,
-         "same, gather, multiply and segmented sum fused in one kernel"),
-        ("spmv_fused_graph", spmvf, spmv_in, os.path.join(FUSED_DIR, "spmv.so"), True, 10,
+        ("spmv_fused", spmvf, spmv_in, generated("spmv", "so"), False, 10,
+         "same, fused by neslgpu_amd.fusion.fuser: gather, multiply and segmented sum in one kernel"),
+        ("spmv_fused_graph", spmvf, spmv_in, generated("spmv", "so"), True, 10,
          "same, fused block replayed as a HIP graph"),
     ], rows
 
@@ -318,7 +289,8 @@ def programs(reps: int, only=()) -> dict:
             r = V.run(text, fused=fused, graphs=graphs, stdin=stdin, arena_units=(1 << 31) - 1, timeout=300)
             ts.append(program_timer(r.stdout) / k * 1e3)
             first = first or r.stdout.splitlines()[:-1]
-        rec = {"ms": round(float(np.median(ts)), 4), "runs_ms": [round(t, 4) for t in ts], "workload": desc}
+        rec = {"ms": round(float(np.median(ts)), 4), "runs_ms": [round(t, 4) for t in ts], "workload": desc,
+               "output": first}
         if name.startswith("spmv"):
             byts = nnz * 12 + ncols * 8 + m * (4 + 8)
             rec["ms_per_product"] = rec.pop("ms")
@@ -343,8 +315,32 @@ def cpu_programs() -> dict:
         if fused:
             continue                              # fused libraries are HIP code
         r = V.run(text, backend=serial, stdin=stdin, arena_units=(1 << 31) - 1, timeout=600)
-        out[name] = {"ms" if k == 1 else "ms_per_product": round(program_timer(r.stdout) / k * 1e3, 3)}
+        out[name] = {"ms" if k == 1 else "ms_per_product": round(program_timer(r.stdout) / k * 1e3, 3),
+                     "output": r.stdout.splitlines()[:-1]}
     return out
+
+
+def compare_outputs(gpu: dict, cpu: dict) -> None:
+    """Marks each GPU program record with matches_serial: the values the
+    program printed (its timer aside) against the serial CVL's in the same
+    run -- equal text for integer results (quicksort's keys, quickhull's
+    hull), and for SpMV's sum of y a relative difference of at most 1e-12
+    (the sums differ in association only).  The output text itself is
+    dropped from the record."""
+    for name, rec in gpu.items():
+        got = rec.pop("output", None)
+        ref = cpu.get(name.replace("_fused_graph", "").replace("_fused", ""), {}).get("output")
+        if got is None or ref is None:
+            continue
+        if name.startswith("spmv"):
+            a, b = float(got[-1].strip("() ")), float(ref[-1].strip("() "))
+            rec["serial_sum_y"] = ref[-1]
+            rec["matches_serial"] = abs(a - b) <= 1e-12 * max(1.0, abs(b))
+        else:
+            rec["matches_serial"] = got == ref
+    for rec in cpu.values():
+        if isinstance(rec, dict):
+            rec.pop("output", None)
 
 
 def measured_traffic(primitive: str):
@@ -473,6 +469,14 @@ def main():
         prims[k] = {"ms": round(t, 4), "GB/s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                     "bytes": byts[k]}
     dom = max(prims, key=lambda k: prims[k]["ms"])
+    extra = {}
+    if "pk2_luz" in names:
+        # the whole VCODE PACK (ref nesl/vcode/main.c:577-642): pk1_luv
+        # counts the flags and returns the count to the host, then pk2_luz
+        t = time_gpu(lambda: sw.run("pack_full"), a.steps, torch)
+        gbs = byts["pack_full"] / t / 1e6
+        extra["pack_full"] = {"ms": round(t, 4), "GB/s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                              "bytes": byts["pack_full"], "calls": "pk1_luv + pk2_luz (count read by the host)"}
     kernel_of = {"add_suz": "k_scan (AddZ, unsegmented)", "max_ruz": "k_reduce_strided + k_reduce_finish (MaxZ)",
                  "add_sez": "k_scan (AddZ, segmented)", "pk2_luz": "k_pack<int>",
                  "smp_puz": "k_part_hist + k_tile_scan + k_partition<int> x2 (pairs, then 16-bit offsets) "
@@ -512,6 +516,7 @@ def main():
                    "n": sw.n, "segments": sw.m, "rank_keys": sw.nr, "primitives_run": names},
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
         "primitives": prims,
+        "outside_sweep": extra,
         "roofline": roof,
     }
     if not a.no_irregular and not a.only:
@@ -522,8 +527,11 @@ def main():
         out["programs"] = programs(3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
-        if not a.no_programs and not a.only:
+        if not a.no_programs and not a.only and not a.programs:
             out["cpu_baseline"]["programs"] = cpu_programs()
+            compare_outputs(out["programs"], out["cpu_baseline"]["programs"])
+    for rec in out.get("programs", {}).values():
+        rec.pop("output", None)
     print(json.dumps(out), flush=True)
 
 

@@ -69,10 +69,12 @@ void cvlhip_fill_splitmix_f64(vec_p d, long long n, unsigned long long seed, dou
 
 /* ---- segmented-op schedules ---------------------------------------- */
 /* A segmented scan / reduction can run under several schedules ("arms"):
- * 0 flat 16-row tiles, 1 flat 8-row tiles, 2 thread per segment, 3 wave per
- * segment, 4 block (256) per segment, 5 block (1024) per segment.  op: 0 +,
- * 1 *, 2 min, 3 max; ty: 0 int, 1 double.  Same arguments and results as the
- * CVL functions (e.g. add_sed / max_red); scratch: cvlhip_seg_scratch units. */
+ * 0 flat look-back tiles of 8 rows per wave, 1 of 2 rows, 2 of 1 row,
+ * 3 thread per segment, 4 wave per segment, 5 workgroup per segment.  The
+ * CVL entries (add_sez, max_red, ...) run the arm mke_fov picked for their
+ * descriptor (cvlhip_segd_pick).  The calls below run the arm given: op 0 +,
+ * 1 *, 2 min, 3 max; ty 0 int, 1 double; same arguments and results as the
+ * CVL functions; scratch: cvlhip_seg_scratch units. */
 int cvlhip_seg_arms(void);
 void cvlhip_seg_scan(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
 void cvlhip_seg_reduce(int op, int ty, int arm, vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
@@ -85,9 +87,26 @@ int cvlhip_seg_scratch(int n, int m);
  * scan differs from add_sed only in the association of its float sums. */
 void cvlhip_add_sed_max_red(vec_p scan_d, vec_p red_d, vec_p s, vec_p sd, int n, int m, vec_p scratch);
 int cvlhip_add_sed_max_red_scratch(int n, int m);
-/* Descriptor statistics for the dispatcher (host floats, see segstrat.hip). */
+/* Descriptor statistics for the dispatcher: the features its network reads
+ * (host floats, cvlhip_seg_features_count of them; the last two, 8-byte
+ * values and reduction, are left 0 for the caller to set). */
 int cvlhip_seg_features_count(void);
 void cvlhip_segd_features(vec_p sd, int n, int m, float *out);
+/* The arm a descriptor's pick names for a scan (reduce 0) or reduction of
+ * elem_bytes-byte values, or -1 when it holds none (fewer than
+ * cvlhip_sched_min_n elements, or built with the dispatcher off). */
+int cvlhip_segd_pick(vec_p sd, int reduce, int elem_bytes);
+int cvlhip_sched_min_n(void);
+/* The network mke_fov runs: the library is built with the trained one
+ * (neslgpu_amd/dispatch_model.npz); set_params installs another (device
+ * pointer, the P floats of cvlhip_mlp_dims) for the descriptors built
+ * after it, or none (NULL); get_params copies the current one out (device
+ * pointer). */
+void cvlhip_sched_set_params(const float *params);
+void cvlhip_sched_get_params(float *params);
+int cvlhip_sched_has_model(void);
+/* 0: mke_fov picks nothing (every segmented op runs arm 0); 1: default. */
+void cvlhip_sched_enable(int on);
 
 /* ---- learned dispatcher network (device pointers, fp32) -------------- */
 /* Sizes: F input features, H hidden units (two ReLU layers), A outputs (one

@@ -293,6 +293,7 @@ __device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
 // ------------------------------------------------------------------ fills
 void fill_u32(void *p, uint32_t v, int64_t count);   // stream-ordered fill kernel
 void zero2_u32(void *p1, int64_t n1, void *p2, int64_t n2);   // two regions, one launch
+void zero3_u32(void *p1, int64_t n1, void *p2, int64_t n2, void *p3, int64_t n3);
 
 // ------------------------------------------------------------------ scalar results to the host
 // A CVL call that returns a scalar (a reduction, ext_vu*) has its kernel

@@ -9,13 +9,10 @@
 // measurement).  Activations for the backward pass are kept in `acts`
 // (B*H for h1, then B*H for h2).
 #include "cvl_hip.hpp"
+#include "mlp.hpp"
 #include "../../include/cvl_hip_ext.h"
 
 namespace cvlhip {
-
-constexpr int kF = 47, kH = 32, kA = 6;
-constexpr int kW1 = 0, kB1 = kW1 + kH * kF, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
-              kB3 = kW3 + kA * kH, kParams = kB3 + kA;
 
 // Y[r][o] = act(b[o] + sum_i X[r][i] W[o][i])
 __global__ void k_dense_fwd(const float *__restrict__ X, const float *__restrict__ W, const float *__restrict__ b,

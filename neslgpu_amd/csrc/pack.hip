@@ -192,7 +192,7 @@ void pk1_lev(vec_p ds, vec_p f, vec_p sd, int n, int m, vec_p scratch) {
 int pk1_lev_scratch(int n, int m) {
     (void)m;
     int64_t a = scan_scratch_bytes<AddZ>(n), b = kReduceScratchBytes;
-    const int64_t c = seg_reduce_scratch_bytes(n, 4, 8);    // pk1_lev: 8-row reduction tiles
+    const int64_t c = seg_reduce_scratch_bytes(n, 4);       // pk1_lev: the segmented count
     if (c > a) a = c;
     return units_for_bytes(a > b ? a : b);
 }

@@ -48,10 +48,15 @@ void hip_fail(hipError_t e, const char *expr, const char *file, int line) {
     if (g_fatal) fatal("%s", g_errbuf);
 }
 
+// Allocated and cleared by the first call, which the library makes in
+// cvlhip_set_stream, CVL_init, alo_foz and cvlhip_capture_begin: hosts
+// start with one of those, so the clear is never recorded into a captured
+// graph.  The clear is synchronous (hipMemset), before any kernel can read
+// the word.
 unsigned *device_error_word() {
     if (!g_dev_err) {
         HIPCHECK(hipMalloc(&g_dev_err, 256));
-        HIPCHECK(hipMemsetAsync(g_dev_err, 0, 256, g_stream));
+        HIPCHECK(hipMemset(g_dev_err, 0, 256));
     }
     return g_dev_err;
 }
@@ -108,25 +113,31 @@ void fill_u32(void *p, uint32_t v, int64_t count) {
     LAUNCHCHECK();
 }
 
-// Two regions cleared in one launch (element i < n1 of the first, else
-// element i - n1 of the second): a launch saved where both are small.
-__global__ void k_zero2_u32(uint32_t *__restrict__ p1, int64_t n1, uint32_t *__restrict__ p2, int64_t n2) {
+// Up to three regions cleared in one launch (element i < n1 of the first,
+// else of the second, else of the third): a launch saved where they are
+// small.
+__global__ void k_zero3_u32(uint32_t *__restrict__ p1, int64_t n1, uint32_t *__restrict__ p2, int64_t n2,
+                            uint32_t *__restrict__ p3, int64_t n3) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2; i += stride) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2 + n3; i += stride) {
         if (i < n1) p1[i] = 0u;
-        else p2[i - n1] = 0u;
+        else if (i < n1 + n2) p2[i - n1] = 0u;
+        else p3[i - n1 - n2] = 0u;
     }
 }
 
-void zero2_u32(void *p1, int64_t n1, void *p2, int64_t n2) {
-    if (n1 < 0) n1 = 0;
-    if (n2 < 0) n2 = 0;
-    if (n1 + n2 <= 0) return;
-    int g = grid_for(n1 + n2, kBlock * 4);
+void zero3_u32(void *p1, int64_t n1, void *p2, int64_t n2, void *p3, int64_t n3) {
+    if (n1 < 0 || !p1) n1 = 0;
+    if (n2 < 0 || !p2) n2 = 0;
+    if (n3 < 0 || !p3) n3 = 0;
+    if (n1 + n2 + n3 <= 0) return;
+    int g = grid_for(n1 + n2 + n3, kBlock * 4);
     if (g > 8192) g = 8192;
-    k_zero2_u32<<<g, kBlock, 0, g_stream>>>((uint32_t *)p1, n1, (uint32_t *)p2, n2);
+    k_zero3_u32<<<g, kBlock, 0, g_stream>>>((uint32_t *)p1, n1, (uint32_t *)p2, n2, (uint32_t *)p3, n3);
     LAUNCHCHECK();
 }
+
+void zero2_u32(void *p1, int64_t n1, void *p2, int64_t n2) { zero3_u32(p1, n1, p2, n2, nullptr, 0); }
 
 // Overlap-safe move of `bytes` bytes (ref facilt.c:161-183 walks forward
 // when dest < src and backward otherwise).  Each block copies a chunk that
@@ -173,7 +184,10 @@ using namespace cvlhip;
 // ------------------------------------------------------------------ extension API
 extern "C" {
 
-void cvlhip_set_stream(void *s) { g_stream = (hipStream_t)s; }
+void cvlhip_set_stream(void *s) {
+    device_error_word();
+    g_stream = (hipStream_t)s;
+}
 void *cvlhip_get_stream(void) { return (void *)g_stream; }
 void cvlhip_set_error_mode(int fatal_on_error) { g_fatal = fatal_on_error; }
 
@@ -209,6 +223,7 @@ void *cvlhip_stream_create(void) {
 
 int cvlhip_capture_begin(void) {
     if (g_stream == nullptr) return -1;                // the null stream cannot be captured
+    device_error_word();                               // allocated outside the capture
     return hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal) == hipSuccess ? 0 : -1;
 }
 
@@ -254,6 +269,7 @@ int siz_fod(int length) { return units_for_bytes((int64_t)length * 8); }
 vec_p alo_foz(int size) {
     void *p = nullptr;
     if (size < 0) return nullptr;
+    device_error_word();
     if (hipMalloc(&p, (size_t)(size > 0 ? size : 1) * kUnit) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;   // the caller reports failure (facilt.c:149-153)

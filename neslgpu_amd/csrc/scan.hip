@@ -16,6 +16,7 @@
 #include "segd.hpp"
 #include "scan.hpp"
 #include "scan_kernel.hpp"
+#include "segsched.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
@@ -158,13 +159,14 @@ constexpr int scan_block() {
 template <class Op, bool SEG, bool REDUCE>
 constexpr int scan_rows() { return REDUCE ? kScanRows : ((sizeof(typename Op::T) == 4 && !SEG) ? 12 : 8); }
 
+// err: the device error word the kernels raise (null: the library's).
 template <class Op, int R, bool SEG, bool REDUCE, int B = scan_block<Op, SEG, REDUCE>()>
-static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *scratch, unsigned *err = nullptr) {
     using S = ScanShape<Op, R, B>;
     if (n <= 0) return;
-    if (!scratch) usage_error("scan of %lld elements called without scratch", (long long)n);
+    if (!scratch) { usage_error("scan of %lld elements called without scratch", (long long)n); return; }
     int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    if (tiles > 0x7fffffff) usage_error("vector too long");
+    if (tiles > 0x7fffffff) { usage_error("vector too long"); return; }
     // one tile needs neither the counter nor cleared status words
     if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<typename Op::T>::kWords * 8) / 4);
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
@@ -172,62 +174,54 @@ static void launch_scan_r(const void *s, void *d, int64_t n, SegBits sb, void *s
     int aligned = al16(s) && al16(d);
     k_scan<Op, R, SEG, REDUCE, ArrayIn<typename Op::T>, B><<<(unsigned)tiles, B, 0, g_stream>>>(
         ArrayIn<typename Op::T>{(const typename Op::T *)s}, (typename Op::T *)d, n, sb, status, counter, aligned,
-        device_error_word());
+        err ? err : device_error_word());
     LAUNCHCHECK();
 }
 
 // The CVL entry points: scratch_bytes counts tiles of kBlock threads and
-// kScanRows rows, so the tiles launched must not be smaller.  (The 8-row
-// schedules of the segmented-op dispatcher get twice that scratch.)
+// kScanRows rows, so the tiles launched must not be smaller.
 template <class Op, bool SEG, bool REDUCE>
-static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+static void launch_scan(const void *s, void *d, int64_t n, SegBits sb, void *scratch, unsigned *err = nullptr) {
     constexpr int R = scan_rows<Op, SEG, REDUCE>();
     static_assert(scan_block<Op, SEG, REDUCE>() * R >= kBlock * kScanRows, "scratch sizes count the most tiles");
-    launch_scan_r<Op, R, SEG, REDUCE>(s, d, n, sb, scratch);
+    launch_scan_r<Op, R, SEG, REDUCE>(s, d, n, sb, scratch, err);
 }
 
-// Segmented reduction in two launches and no look-back: every tile reduces
-// its segments with an empty incoming prefix and records its aggregate and
-// the partial of the segment entering it (NolbRecords after the scratch
-// header); one workgroup then folds the tile aggregates in order and
-// completes those segments.  Only a segment that crosses a tile boundary
-// needs the second step, so the data is read once and no tile waits on
-// another (max_red of 2^26 doubles: 0.26 ms with a look-back).
-template <class Op, int R>
-static void launch_seg_reduce(const void *s, void *d, int64_t n, SegBits sb, void *scratch) {
+// Scratch of a segmented reduction of n elements of elem_bytes each under
+// any schedule (segsched.hpp).
+int64_t seg_reduce_scratch_bytes(int64_t n, int elem_bytes) {
+    return elem_bytes == 8 ? sched_scratch_bytes<AddD, true>(n) : sched_scratch_bytes<AddZ, true>(n);
+}
+
+template <class Op>
+void scan_unseg(void *d, const void *s, int64_t n, void *scratch, unsigned *err) {
+    launch_scan<Op, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch, err);
+}
+
+// Segmented scan.  Vectors shorter than kSchedMinN (their descriptors carry
+// no pick) run the fixed schedule one tile per workgroup; longer ones run
+// the schedule the descriptor names (k_seg_sched), or `arm` when one is
+// forced.  Scratch: the larger of scan_scratch_bytes and
+// sched_scratch_bytes<Op, false>.
+template <class Op>
+void scan_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch, int arm,
+              unsigned *err) {
     using T = typename Op::T;
-    using S = ScanShape<Op, R, kSegReduceBlock>;
-    if (n <= 0) return;
-    if (!scratch) usage_error("segmented reduction of %lld elements called without scratch", (long long)n);
-    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    if (tiles > 0x7fffffff) usage_error("vector too long");
-    void *recs = (char *)scratch + kScanHeaderBytes;
-    k_scan<Op, R, true, true, ArrayIn<T>, kSegReduceBlock, true><<<(unsigned)tiles, kSegReduceBlock, 0, g_stream>>>(
-        ArrayIn<T>{(const T *)s}, (T *)d, n, sb, (uint64_t *)recs, nullptr, al16(s), device_error_word());
-    LAUNCHCHECK();
-    if (tiles > 1) {
-        k_seg_reduce_fix<Op><<<1, kFixThreads, 0, g_stream>>>(NolbRecords<T>::at(recs, tiles), tiles, (T *)d);
-        LAUNCHCHECK();
-    }
-}
-
-int64_t seg_reduce_scratch_bytes(int64_t n, int elem_bytes, int rows) {
-    const int64_t tile = (int64_t)(kBlock / kWave) * rows * kWave * (16 / elem_bytes);
-    const int64_t tiles = (n + tile - 1) / tile;
-    const int64_t recs = elem_bytes == 8 ? NolbRecords<double>::bytes(tiles) : NolbRecords<int>::bytes(tiles);
-    return kScanHeaderBytes + recs + 256;
-}
-
-template <class Op>
-void scan_unseg(void *d, const void *s, int64_t n, void *scratch) {
-    launch_scan<Op, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch);
-}
-
-template <class Op>
-void scan_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    if (m == 1) { scan_unseg<Op>(d, s, n, scratch); return; }   // vprims.c:75
+    if (m == 1) { scan_unseg<Op>(d, s, n, scratch, err); return; }   // vprims.c:75
     SegdView v(sd, n, m);
-    launch_scan<Op, true, false>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
+    if (arm < 0 && n < kSchedMinN) {
+        launch_scan<Op, true, false>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch, err);
+        return;
+    }
+    if (n <= 0) return;
+    if (!scratch) { usage_error("segmented scan of %lld elements called without scratch", (long long)n); return; }
+    constexpr int64_t tile1 = sched_tile<Op, false, 1>();
+    const int64_t tiles1 = (n + tile1 - 1) / tile1;
+    fill_u32(scratch, 0, (kSchedHeader + tiles1 * TileStatus<T>::kWords * 8) / 4);
+    k_seg_sched<Op, false, ArrayIn<T>><<<sched_grid<Op, false, ArrayIn<T>>(n), sched_block<Op, false>(), 0, g_stream>>>(
+        ArrayIn<T>{(const T *)s}, (T *)d, n, v, sd, 0, scratch, al16(s) && al16(d), err ? err : device_error_word(),
+        arm);
+    LAUNCHCHECK();
 }
 
 // Unsegmented reduction for operators whose result does not depend on the
@@ -337,57 +331,26 @@ typename Op::T reduce_unseg(const void *s, int64_t n, void *scratch) {
     return scalar_from_bits<T>(mailbox_wait(seq));
 }
 
-// Segmented reduction in tiles of R rows per wave: identities for the empty
-// segments, then the two-step reduction.  The CVL entry takes 8 rows (2^26
-// doubles in Zipf(1.5) segments: max_red 0.229 -> 0.206 ms against 16; 2^28
-// ints in geometric(256) segments: max_rez 0.575 -> 0.509 ms).
-template <class Op, int R>
-static void reduce_seg_rows(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
+// Segmented reduction: identities for the empty segments, then the
+// schedule of seg_reduce_run (segsched.hpp): the fixed two-step reduction
+// below kSchedMinN elements, else the descriptor's pick or the forced arm.
+template <class Op>
+void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch, int arm) {
     using T = typename Op::T;
     if (m <= 0) return;
     if (m == 1) {                                    // vprims.c:167
         reduce_unseg_dev<Op>(d, s, n, scratch);
         return;
     }
+    if (!scratch) { usage_error("segmented reduction of %lld elements called without scratch", (long long)n); return; }
     SegdView v(sd, n, m);
     int g = grid_for(m, kBlock);
     if (g > 4096) g = 4096;
     k_empty_identity<T><<<g, kBlock, 0, g_stream>>>((T *)d, v.lens, m, Op::cvl_id());
     LAUNCHCHECK();
-    launch_seg_reduce<Op, R>(s, d, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, scratch);
-}
-
-template <class Op>
-void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    reduce_seg_rows<Op, kSegReduceRows>(d, s, sd, n, m, scratch);
-}
-
-// Other schedules the segmented-op dispatcher may pick: 16-row scan tiles
-// of 256 threads, 8-row scan tiles in workgroups of the default size (the
-// scratch they need is twice that of the CVL entries) and 16-row reduction
-// tiles.
-template <class Op>
-void scan_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    if (m == 1) { launch_scan_r<Op, 16, false, false, kBlock>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
-    SegdView v(sd, n, m);
-    launch_scan_r<Op, 16, true, false, kBlock>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
-}
-
-template <class Op>
-void scan_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    if (m == 1) { launch_scan_r<Op, 8, false, false>(s, d, n, SegBits{nullptr, nullptr, nullptr}, scratch); return; }
-    SegdView v(sd, n, m);
-    launch_scan_r<Op, 8, true, false>(s, d, n, SegBits{v.heads, v.wcnt, v.nz}, scratch);
-}
-
-template <class Op>
-void reduce_seg_r8(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    reduce_seg_rows<Op, 8>(d, s, sd, n, m, scratch);
-}
-
-template <class Op>
-void reduce_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m, void *scratch) {
-    reduce_seg_rows<Op, 16>(d, s, sd, n, m, scratch);
+    seg_reduce_run<Op, ArrayIn<T>>(ArrayIn<T>{(const T *)s}, (T *)d, n, v, sd, false, scratch, al16(s),
+                                   device_error_word(), arm, g_stream);
+    LAUNCHCHECK();
 }
 
 // add_sed and max_red of the same values in one pass (one read of s): the
@@ -400,16 +363,17 @@ void reduce_seg_r16(void *d, const void *s, const void *sd, int64_t n, int64_t m
 constexpr int kPairRows = CVLHIP_PAIR_ROWS;
 using AddMaxD = PairOp<AddD, MaxD>;
 
-int64_t add_max_pair_scratch_bytes(int64_t n) {
-    using S = ScanShape<AddMaxD, kPairRows>;
-    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    return kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8 + 256;
-}
-
 #ifndef CVLHIP_PAIR_BLOCK
 #define CVLHIP_PAIR_BLOCK 256
 #endif
 constexpr int kPairBlock = CVLHIP_PAIR_BLOCK;
+
+// Counts the tiles of the shape the pass is launched with.
+int64_t add_max_pair_scratch_bytes(int64_t n) {
+    using S = ScanShape<AddMaxD, kPairRows, kPairBlock>;
+    const int64_t tiles = (n + S::TILE - 1) / S::TILE;
+    return kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8 + 256;
+}
 
 void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd, int64_t n, int64_t m,
                   void *scratch) {
@@ -434,12 +398,8 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
 }
 
 #define CVLHIP_INST_SEG(OP)                                                                         \
-    template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);      \
-    template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *);    \
-    template void scan_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *);   \
-    template void scan_seg_r16<OP>(void *, const void *, const void *, int64_t, int64_t, void *);  \
-    template void reduce_seg_r8<OP>(void *, const void *, const void *, int64_t, int64_t, void *); \
-    template void reduce_seg_r16<OP>(void *, const void *, const void *, int64_t, int64_t, void *);
+    template void scan_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *, int, unsigned *); \
+    template void reduce_seg<OP>(void *, const void *, const void *, int64_t, int64_t, void *, int);
 CVLHIP_INST_SEG(AddZ)
 CVLHIP_INST_SEG(MulZ)
 CVLHIP_INST_SEG(MinZ)
@@ -448,12 +408,12 @@ CVLHIP_INST_SEG(AddD)
 CVLHIP_INST_SEG(MulD)
 CVLHIP_INST_SEG(MinD)
 CVLHIP_INST_SEG(MaxD)
+CVLHIP_INST_SEG(CountB)
 template int64_t scan_scratch_bytes<AddD>(int64_t);
 
 // Explicit instantiations used by other translation units.
-template void scan_unseg<AddZ>(void *, const void *, int64_t, void *);
-template void scan_unseg<CountB>(void *, const void *, int64_t, void *);
-template void reduce_seg<CountB>(void *, const void *, const void *, int64_t, int64_t, void *);
+template void scan_unseg<AddZ>(void *, const void *, int64_t, void *, unsigned *);
+template void scan_unseg<CountB>(void *, const void *, int64_t, void *, unsigned *);
 template CountB::T reduce_unseg<CountB>(const void *, int64_t, void *);
 template int64_t scan_scratch_bytes<AddZ>(int64_t);
 template AddZ::T reduce_unseg<AddZ>(const void *, int64_t, void *);
@@ -470,24 +430,23 @@ using namespace cvlhip;
     int OPN##_su##TY##_scratch(int len) { return units_for_bytes(scan_scratch_bytes<OP>(len)); } \
     unsigned int OPN##_su##TY##_inplace(void) { return INPLACE_1; }                          \
     void OPN##_se##TY(vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch) {             \
-        scan_seg<OP>(d, s, sd, n, m, scratch);                                              \
+        scan_seg<OP>(d, s, sd, n, m, scratch, -1);                                          \
     }                                                                                       \
     int OPN##_se##TY##_scratch(int n, int m) {                                               \
         (void)m;                                                                            \
-        return units_for_bytes(scan_scratch_bytes<OP>(n));                                  \
+        const int64_t a = scan_scratch_bytes<OP>(n), b = sched_scratch_bytes<OP, false>(n); \
+        return units_for_bytes(a > b ? a : b);                                              \
     }                                                                                       \
     unsigned int OPN##_se##TY##_inplace(void) { return INPLACE_1; }                          \
     OP::T OPN##_ru##TY(vec_p s, int len, vec_p scratch) { return reduce_unseg<OP>(s, len, scratch); } \
     int OPN##_ru##TY##_scratch(int len) { return units_for_bytes(reduce_scratch_bytes<OP>(len)); } \
     unsigned int OPN##_ru##TY##_inplace(void) { return INPLACE_NONE; }                       \
     void OPN##_re##TY(vec_p d, vec_p s, vec_p sd, int n, int m, vec_p scratch) {             \
-        reduce_seg<OP>(d, s, sd, n, m, scratch);                                            \
+        reduce_seg<OP>(d, s, sd, n, m, scratch, -1);                                        \
     }                                                                                       \
     int OPN##_re##TY##_scratch(int n, int m) {                                               \
         (void)m;                                                                            \
-        int64_t a = scan_scratch_bytes<OP>(n), b = reduce_scratch_bytes<OP>(n);             \
-        const int64_t c = seg_reduce_scratch_bytes(n, (int)sizeof(OP::T), kSegReduceRows);  \
-        if (c > a) a = c;                                                                   \
+        const int64_t a = sched_scratch_bytes<OP, true>(n), b = reduce_scratch_bytes<OP>(n); \
         return units_for_bytes(a > b ? a : b);                                              \
     }                                                                                       \
     unsigned int OPN##_re##TY##_inplace(void) { return INPLACE_NONE; }

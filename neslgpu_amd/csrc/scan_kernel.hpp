@@ -202,11 +202,14 @@ struct NolbRecords {
 // NOLB (segmented reductions only): no look-back; `status` holds the
 // NolbRecords of gridDim.x tiles and k_seg_reduce_fix finishes the
 // segments that cross tiles.
-template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock,
-          bool NOLB = false>
-__global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
-                                            uint64_t *__restrict__ status, unsigned *__restrict__ counter,
-                                            int aligned_i, unsigned *__restrict__ err) {
+//
+// scan_tile is the work of one workgroup on one tile (`tile` of `ntiles`;
+// NOLB records are laid out for ntiles tiles).  k_scan runs one tile per
+// workgroup; the segmented-op schedules (k_seg_sched) loop over tiles.
+template <class Op, int R, bool SEG, bool REDUCE, class In, int B, bool NOLB>
+__device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
+                                          uint64_t *__restrict__ status, int64_t tile, int64_t ntiles,
+                                          bool aligned, unsigned *__restrict__ err) {
     using T = typename Op::T;
     using S = ScanShape<Op, R, B>;
     using St = TileStatus<T>;
@@ -216,9 +219,7 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     static_assert(!DUAL || (SEG && REDUCE), "a dual pass is a segmented scan and reduction");
     static_assert(!NOLB || (SEG && REDUCE && !DUAL), "only segmented reductions run without look-back");
     const Op op;
-    const bool aligned = aligned_i != 0;
 
-    __shared__ unsigned s_tile;
     __shared__ T s_wagg[S::WAVES];
     __shared__ int s_wflag[S::WAVES];
     __shared__ T s_win[S::WAVES];
@@ -226,10 +227,7 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     __shared__ int s_fix_slot;         // NOLB: this tile's record of the segment entering it
     __shared__ T s_fix_val;
 
-    // tile ids in launch order (look-back waits only on started tiles); a
-    // single-tile launch passes no counter
-    if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
-    if (NOLB && threadIdx.x == 0) {
+    if (NOLB && threadIdx.x == 0) {    // read by thread 0 only after the barriers below
         s_fix_slot = -1;
         s_fix_val = Op::neutral();
     }
@@ -242,8 +240,6 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
             sb.m = v.m;
         }
     }
-    __syncthreads();
-    const int64_t tile = s_tile;
     const int w = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int64_t wbase = tile * S::TILE + (int64_t)w * S::WAVE_ELEMS + (int64_t)lane * V;
@@ -360,7 +356,7 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
         T P = Op::neutral();
         if constexpr (NOLB) {
             if (lane == 0) {
-                const NolbRecords<T> rec = NolbRecords<T>::at(status, gridDim.x);
+                const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
                 rec.agg[tile] = tv;
                 rec.flag[tile] = tf;
             }
@@ -435,11 +431,25 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     if constexpr (NOLB) {
         __syncthreads();
         if (threadIdx.x == 0) {
-            const NolbRecords<T> rec = NolbRecords<T>::at(status, gridDim.x);
+            const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
             rec.slot[tile] = s_fix_slot;
             rec.val[tile] = s_fix_val;
         }
     }
+}
+
+template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock,
+          bool NOLB = false>
+__global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
+                                            uint64_t *__restrict__ status, unsigned *__restrict__ counter,
+                                            int aligned_i, unsigned *__restrict__ err) {
+    __shared__ unsigned s_tile;
+    // tile ids in launch order (look-back waits only on started tiles); a
+    // single-tile launch passes no counter
+    if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
+    __syncthreads();
+    scan_tile<Op, R, SEG, REDUCE, In, B, NOLB>(in, out, n, sb, status, (int64_t)s_tile, (int64_t)gridDim.x,
+                                               aligned_i != 0, err);
 }
 
 // Rows per wave and threads per workgroup of the tiles of a segmented
@@ -459,8 +469,8 @@ constexpr int kSegReduceBlock = CVLHIP_SEG_REDUCE_BLOCK;
 // out[slot[t]] = cvl_id op (prefix(t) op val[t]).
 constexpr int kFixThreads = 1024;
 template <class Op>
-__global__ __launch_bounds__(kFixThreads) void k_seg_reduce_fix(NolbRecords<typename Op::T> rec, int64_t g,
-                                                                typename Op::T *__restrict__ out) {
+__device__ __forceinline__ void seg_reduce_fix_body(NolbRecords<typename Op::T> rec, int64_t g,
+                                                    typename Op::T *__restrict__ out) {
     using T = typename Op::T;
     constexpr int W = kFixThreads / kWave;
     const Op op;
@@ -534,6 +544,12 @@ __global__ __launch_bounds__(kFixThreads) void k_seg_reduce_fix(NolbRecords<type
             }
         }
     }
+}
+
+template <class Op>
+__global__ __launch_bounds__(kFixThreads) void k_seg_reduce_fix(NolbRecords<typename Op::T> rec, int64_t g,
+                                                                typename Op::T *__restrict__ out) {
+    seg_reduce_fix_body<Op>(rec, g, out);
 }
 
 

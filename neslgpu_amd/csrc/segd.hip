@@ -11,9 +11,61 @@
 #include "segd.hpp"
 #include "scan.hpp"
 #include "lookback.hpp"
+#include "mlp.hpp"
 #include "../../include/cvl.h"
+#include "../../include/cvl_hip_ext.h"
+
+#include <stddef.h>
+
+// The trained dispatcher network, generated at build time from
+// neslgpu_amd/dispatch_model.npz (sched_model.inc in the build directory).
+#if __has_include("sched_model.inc")
+#include "sched_model.inc"
+#endif
 
 namespace cvlhip {
+
+// ---- the dispatcher's pick
+// Parameters of the network that picks a schedule for every descriptor of
+// at least kSchedMinN elements (mlp.hpp for the layout).
+#ifdef CVLHIP_SCHED_MODEL_PARAMS
+__device__ float g_sched_params[kParams] = {CVLHIP_SCHED_MODEL_PARAMS};
+static int g_sched_model = 1;
+#else
+__device__ float g_sched_params[kParams];
+static int g_sched_model = 0;
+#endif
+static int g_sched_enabled = 1;
+
+constexpr int kHist = 2 * kLenClasses;
+
+__device__ __forceinline__ void hist_add(unsigned *h, unsigned len) {
+    if (len == 0) return;
+    int c = 31 - __clz(len);
+    if (c >= kLenClasses) c = kLenClasses - 1;
+    atomicAdd(h + c, len);
+    atomicAdd(h + kLenClasses + c, 1u);
+}
+
+// Waves 0..3 of a workgroup of at least 256 threads each pick the schedule
+// of one kind of operation (sched_slot order) from the histogram `hist`;
+// thread 0 stores the picks in the header.  Every thread of the workgroup
+// must call it.
+__device__ void sched_pick_block(SegdHeader *hdr, int64_t n, int64_t m, const unsigned *hist, unsigned *s_pick) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (w < 4) {
+        const bool red = (w >> 1) != 0, dbl = (w & 1) != 0;
+        const float f = sched_feature(lane < kF ? lane : 0, n, m, hist, hist + kLenClasses, dbl, red);
+        const int a = mlp_pick_wave(g_sched_params, f);
+        if (lane == 0) s_pick[w] = (unsigned)a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned p = 0;
+        for (int k = 0; k < 4; k++) p |= (s_pick[k] & 15u) << (4 * k);
+        hdr->sched = (kSchedMagic << 16) | p;
+    }
+}
 
 // ---- small descriptors: the whole construction in one workgroup
 // (m and the bitmap's word count at most kSmallSegd), instead of a dozen
@@ -42,13 +94,17 @@ __device__ unsigned block_exclusive_scan(unsigned x, unsigned *s_wave, unsigned 
     return s_wave[w] + inc - x;
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restrict__ lens_in, int64_t *hdr, int *lens,
-                                                               int *offs, int *nz, int *wcnt, uint32_t *heads,
-                                                               int64_t n, int64_t m, int64_t words) {
+__global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restrict__ lens_in, SegdHeader *hdr,
+                                                               int *lens, int *offs, int *nz, int *wcnt,
+                                                               uint32_t *heads, int64_t n, int64_t m, int64_t words,
+                                                               int pick) {
     __shared__ unsigned s_wave[kSmallThreads / kWave];
     __shared__ unsigned s_total;
     __shared__ uint32_t s_heads[kSmallSegd + 8];
+    __shared__ unsigned s_hist[kHist];
+    __shared__ unsigned s_pick[4];
     const int tid = threadIdx.x;
+    if (tid < kHist) s_hist[tid] = 0u;     // ordered by the barriers of the scans below
     unsigned len[kSmallPer], off[kSmallPer], pos[kSmallPer];
     unsigned lsum = 0, psum = 0;
 #pragma unroll
@@ -75,15 +131,20 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_small(const int *__restr
             if ((int)len[k] > 0) {
                 nz[pbase + pos[k]] = (int)g;
                 if (o >= 0 && (int64_t)o < n) atomicOr(&s_heads[o >> 5], 1u << (o & 31));
+                hist_add(s_hist, len[k]);
             }
         }
     }
     if (tid == 0) {
         offs[m] = (int)ltot;
-        hdr[0] = n;
-        hdr[1] = m;
+        hdr->n = n;
+        hdr->m = m;
+        hdr->stats = kStatsMagic;
     }
     __syncthreads();
+    if (tid < kHist) hdr->hist[tid] = s_hist[tid];
+    if (pick) sched_pick_block(hdr, n, m, s_hist, s_pick);
+    else if (tid == 0) hdr->sched = 0u;
     unsigned hc[kSmallPer], csum = 0;
 #pragma unroll
     for (int k = 0; k < kSmallPer; k++) {
@@ -162,7 +223,9 @@ __device__ typename Op::T block_tile_prefix(typename Op::T x, typename Op::T *s_
     return op(s_wave[w], *excl_in_block);
 }
 
-__global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__restrict__ lens_in, int64_t *hdr,
+// The header's histogram is cleared before the launch; every tile adds its
+// segments' counts to it.
+__global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__restrict__ lens_in, SegdHeader *hdr,
                                                                  int *lens, int *offs, int *nz, uint32_t *heads,
                                                                  int64_t n, int64_t m, uint64_t *status,
                                                                  unsigned *counter, unsigned *err) {
@@ -170,7 +233,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
     __shared__ uint64_t s_wave[kSmallThreads / kWave];
     __shared__ uint64_t s_total;
     __shared__ int s_head[kSmallSegd];          // start of each non-empty segment of the tile, else -1
+    __shared__ unsigned s_hist[kHist];
     const int tid = threadIdx.x;
+    if (tid < kHist) s_hist[tid] = 0u;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
@@ -194,7 +259,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
         const uint64_t q = op(P, loc[k]);
         o4[k] = (int)(uint32_t)q;
         s_head[tid * kSmallPer + k] = (g < m && (int)len[k] > 0 && o4[k] >= 0 && (int64_t)o4[k] < n) ? o4[k] : -1;
-        if (g < m && (int)len[k] > 0) nz[q >> 32] = (int)g;
+        if (g < m && (int)len[k] > 0) {
+            nz[q >> 32] = (int)g;
+            hist_add(s_hist, len[k]);
+        }
     }
     static_assert(kSmallPer == 4, "one 16-byte store per thread");
     typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -213,6 +281,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
         }
     }
     __syncthreads();
+    if (tid < kHist && s_hist[tid]) atomicAdd(&hdr->hist[tid], s_hist[tid]);
     // Head bits: wave w takes the tile's segments w*256 .. w*256+255, 64
     // consecutive ones per round, ORs the bits of lanes whose heads share a
     // bitmap word (segment starts increase with the segment index) and
@@ -236,17 +305,22 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
     }
     if (tile == (int64_t)gridDim.x - 1 && tid == 0) {
         offs[m] = (int)(uint32_t)s_total;
-        hdr[0] = n;
-        hdr[1] = m;
+        hdr->n = n;
+        hdr->m = m;
+        hdr->stats = kStatsMagic;
     }
 }
 
+// With `pick`, the workgroup of tile 0 also picks the schedules from the
+// histogram k_segd_offsets completed.
 __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__restrict__ heads, int *wcnt,
                                                               int64_t words, uint64_t *status, unsigned *counter,
-                                                              unsigned *err) {
+                                                              unsigned *err, SegdHeader *hdr, int64_t n, int64_t m,
+                                                              int pick) {
     __shared__ unsigned s_tile;
     __shared__ unsigned s_wave[kSmallThreads / kWave];
     __shared__ unsigned s_total;
+    __shared__ unsigned s_pick[4];
     const int tid = threadIdx.x;
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
@@ -267,12 +341,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__r
         if (wd < words) wcnt[wd] = (int)(P + loc[k]);
     }
     if (tile == (int64_t)gridDim.x - 1 && tid == 0) wcnt[words] = (int)s_total;
+    if (pick && tile == 0) sched_pick_block(hdr, n, m, hdr->hist, s_pick);
 }
 
-__global__ void k_segd_empty(int64_t *hdr, int *offs, int64_t n) {
+__global__ void k_segd_empty(SegdHeader *hdr, int *offs, int64_t n) {
     if (threadIdx.x == 0) {
-        hdr[0] = n;
-        hdr[1] = 0;
+        hdr->n = n;
+        hdr->m = 0;
+        hdr->stats = kStatsMagic;
         offs[0] = 0;
     }
 }
@@ -291,7 +367,8 @@ int64_t segd_build_scratch_bytes(int64_t n, int64_t m) {
 void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scratch) {
     SegdLayout L(n, m);
     char *b = (char *)sd;
-    int64_t *hdr = (int64_t *)(b + L.hdr);
+    SegdHeader *hdr = (SegdHeader *)(b + L.hdr);
+    const int pick = g_sched_enabled && g_sched_model && n >= kSchedMinN;
     int *lens = (int *)(b + L.lens);
     int *offs = (int *)(b + L.offs);
     int *nz = (int *)(b + L.nz);
@@ -301,7 +378,7 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
 
     if (m <= kSmallSegd && L.words <= kSmallSegd) {
         k_segd_small<<<1, kSmallThreads, 0, g_stream>>>((const int *)lens_in, hdr, lens, offs, nz, wcnt, heads, n,
-                                                        m, L.words);
+                                                        m, L.words, pick);
         LAUNCHCHECK();
         return;
     }
@@ -309,7 +386,10 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     unsigned *counters = (unsigned *)scratch;                          // [0] pass 1, [1] pass 2
     uint64_t *st_a = (uint64_t *)((char *)scratch + 256);
     uint64_t *st_b = st_a + 2 * (ta + 1);
-    zero2_u32(scratch, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4, heads, L.words + 8);
+    // scratch counters and status, the bitmap, and the header from its
+    // schedule word to the end of the histogram, in one launch
+    zero3_u32(scratch, (256 + 16 * (ta + 1) + 16 * (tb + 1)) / 4, heads, L.words + 8, &hdr->sched,
+              (int64_t)(sizeof(SegdHeader) - offsetof(SegdHeader, sched)) / 4);
     if (m > 0) {
         k_segd_offsets<<<(unsigned)ta, kSmallThreads, 0, g_stream>>>((const int *)lens_in, hdr, lens, offs, nz, heads,
                                                                      n, m, st_a, counters, device_error_word());
@@ -320,7 +400,7 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
     }
     if (L.words > 0) {
         k_segd_wcnt<<<(unsigned)tb, kSmallThreads, 0, g_stream>>>(heads, wcnt, L.words, st_b, counters + 1,
-                                                                  device_error_word());
+                                                                  device_error_word(), hdr, n, m, pick);
         LAUNCHCHECK();
     } else {
         fill_u32(wcnt, 0, 1);
@@ -332,6 +412,24 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
 using namespace cvlhip;
 
 extern "C" {
+
+// ---- the dispatcher's network in the descriptor build
+void cvlhip_sched_set_params(const float *params) {
+    if (!params) {                    // back to no network: descriptors get no pick
+        g_sched_model = 0;
+        return;
+    }
+    HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_sched_params), params, sizeof(float) * kParams, 0,
+                                    hipMemcpyDeviceToDevice, g_stream));
+    g_sched_model = 1;
+}
+void cvlhip_sched_get_params(float *params) {
+    HIPCHECK(hipMemcpyFromSymbolAsync(params, HIP_SYMBOL(g_sched_params), sizeof(float) * kParams, 0,
+                                      hipMemcpyDeviceToDevice, g_stream));
+}
+int cvlhip_sched_has_model(void) { return g_sched_model; }
+void cvlhip_sched_enable(int on) { g_sched_enabled = on != 0; }
+int cvlhip_sched_min_n(void) { return (int)kSchedMinN; }
 
 int siz_fos(int n, int m) { return units_for_bytes(SegdLayout(n, m).bytes); }
 

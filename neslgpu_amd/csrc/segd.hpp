@@ -6,8 +6,10 @@
 // mke_fov, which the segmented kernels use to find segment boundaries in
 // O(1):
 //
-//   hdr              int64 n, int64 m (a kernel handed only the descriptor,
-//                    such as a generated fused operation, reads its shape)
+//   hdr              SegdHeader: int64 n, int64 m (a kernel handed only the
+//                    descriptor, such as a generated fused operation, reads
+//                    its shape), the length-class histogram and the
+//                    schedules the dispatcher picked for it
 //   lens[m]          segment lengths (int32)
 //   offs[m+1]        exclusive prefix of lens; offs[m] = n
 //   nz[m]            indices of the non-empty segments, in order
@@ -23,6 +25,35 @@
 #include <stdint.h>
 
 namespace cvlhip {
+
+// Segment-length classes of the descriptor statistics: class c holds the
+// lengths in [2^c, 2^(c+1)); the last class is open above.
+constexpr int kLenClasses = 21;
+
+// The first 256 bytes of a descriptor.  mke_fov fills the histogram while
+// it scans the lengths and, for descriptors of at least kSchedMinN
+// elements, lets the dispatcher network pick a schedule for each kind of
+// segmented operation (scan or reduction, 4- or 8-byte values); the picks
+// are 4-bit arm numbers under a magic half-word, so a descriptor that was
+// not built that way reads as "no pick".
+struct SegdHeader {
+    int64_t n, m;
+    uint32_t sched;                  // kSchedMagic << 16 | picks, 4 bits each at 4 * sched_slot
+    uint32_t stats;                  // kStatsMagic when the histogram below is filled
+    // hist[c]: elements in segments of length class c; hist[kLenClasses + c]:
+    // segments of class c (empty segments are not counted)
+    uint32_t hist[2 * kLenClasses];
+};
+static_assert(sizeof(SegdHeader) <= 256, "the header section is 256 bytes");
+constexpr uint32_t kSchedMagic = 0x5c4du;
+constexpr uint32_t kStatsMagic = 0x57a75u;
+// Descriptors of fewer elements get no pick: every schedule then runs in
+// one or two tiles and the fixed one is used.
+constexpr int64_t kSchedMinN = 1 << 15;
+// Where a kind of operation finds its pick: scan/reduction x element size.
+__host__ __device__ constexpr int sched_slot(bool reduce, int elem_bytes) {
+    return (reduce ? 2 : 0) + (elem_bytes == 8 ? 1 : 0);
+}
 
 struct SegdLayout {
     int64_t hdr, lens, offs, nz, wcnt, heads, bytes;   // byte offsets and total size

@@ -1,20 +1,29 @@
 """Scheduler for segmented scans and reductions: exhaustive autotuner and a
 learned dispatcher.
 
-A segmented operation can run under several schedules ("arms", see
-csrc/segstrat.hip): flat look-back tiles of two sizes, thread-, wave- or
-block-per-segment.  The autotuner times every arm on a workload (HIP
-events); the dispatcher is a small MLP that reads device-computed
-descriptor statistics (n, m, log2-length histograms by elements and by
-segments, dtype, scan/reduce) and predicts every arm's log-time; it picks
-the arm with the lowest prediction.  The network's forward and backward
-passes and its Adam steps are HIP kernels of libcvl_hip.so
-(cvlhip_mlp_*); this module only orchestrates them.  Only the schedule
-changes with the choice, never the semantics; the autotuner stays the
-fallback and the judge of the dispatcher's regret.
+A segmented operation can run under several schedules ("arms",
+csrc/segsched.hpp): flat look-back tiles of 8, 2 or 1 rows per wave, or
+thread-, wave- or workgroup-per-segment.  The autotuner times every arm on a
+workload (HIP events).  The dispatcher is a small MLP that reads the
+descriptor statistics mke_fov computes on the device (n, m, log2-length
+histograms by elements and by segments, value size, scan/reduce) and
+predicts every arm's log-time.  The library runs the network itself, in
+mke_fov, with the parameters trained here (built into libcvl_hip.so from
+dispatch_model.npz), and stores the pick in the descriptor; every CVL
+segmented call on that descriptor runs the picked arm, with no host round
+trip.  The network's forward and backward passes and its Adam steps are HIP
+kernels (cvlhip_mlp_*); this module only orchestrates them.  Only the
+schedule changes with the choice, never the semantics; the autotuner stays
+the judge of the dispatcher's regret.
 
-    python -m neslgpu_amd.dispatch train [results.json] [model.npz]
-        # gather timings, train, save the model (default: next to this file), evaluate
+Three workload sets, kept apart:
+  TRAIN  -- timings the network is fitted to;
+  VALID  -- picks the training run (epochs, seed) with the lowest regret;
+  EVAL   -- held out: never used for any choice; its regret is the figure
+            reported.
+
+    python -m neslgpu_amd.dispatch train [record.json] [model.npz]
+        # gather timings, train, save the model, evaluate on EVAL live
 """
 from __future__ import annotations
 
@@ -30,8 +39,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "dispatch_model.npz")
 
-ARMS = ["flat16", "flat8", "thread", "wave", "block256", "block1024"]
+ARMS = ["flat8", "flat2", "flat1", "thread", "wave", "block"]
 OPS = {"add": 0, "mul": 1, "min": 2, "max": 3}
+# (kind, operator, value type): one of each kind of pick a descriptor holds
+KINDS = (("scan", "add", "z"), ("scan", "add", "d"), ("reduce", "max", "z"), ("reduce", "max", "d"))
+CVL_NAME = {("scan", "add", "z"): "add_sez", ("scan", "add", "d"): "add_sed",
+            ("reduce", "max", "z"): "max_rez", ("reduce", "max", "d"): "max_red"}
 
 
 def mlp_dims(lib):
@@ -69,38 +82,34 @@ def lengths(kind: str, n: int, param: float, rng: np.random.Generator, lmax: int
     return L[L > 0].astype(np.int32)
 
 
-TRAIN_WORKLOADS = (
-    [("geometric", m) for m in (2, 4, 16, 64, 256, 1024, 8192)]
-    + [("uniform", m) for m in (3, 24, 200, 3000)]
-    + [("zipf", a) for a in (1.3, 1.7, 2.2, 3.0)]
-    + [("const", c) for c in (1, 32, 100000)]
-    + [("bimodal", 0)]
-)
-EVAL_WORKLOADS = [("zipf", a) for a in (1.2, 1.5, 1.8, 2.5)]
+TRAIN_DISTS = ([("geometric", m) for m in (2, 8, 64, 1024, 8192)]
+               + [("uniform", m) for m in (3, 200, 3000)]
+               + [("zipf", a) for a in (1.3, 1.7, 2.2, 3.0)]
+               + [("const", c) for c in (1, 32, 100000)]
+               + [("bimodal", 0)])
+TRAIN_SIZES = [1 << k for k in (15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25)]
+VALID_DISTS = [("zipf", 1.6), ("geometric", 256), ("uniform", 40)]
+VALID_SIZES = [1 << k for k in (16, 18, 20, 22, 24)]
+EVAL_DISTS = [("zipf", 1.4), ("zipf", 2.0), ("zipf", 2.8), ("geometric", 512)]
+EVAL_SIZES = [1 << k for k in (17, 19, 21, 23, 25)]
 
 
 # ------------------------------------------------------------------ timing
-class Autotuner:
-    """Times every arm of a segmented op (median of `reps` runs, HIP events)."""
+class Timer:
+    """Median time (ms) of a callable on the library's stream, HIP events."""
 
-    def __init__(self, cv, reps: int = 3, slow_ms: float = 50.0):
+    def __init__(self, reps: int = 5, slow_ms: float = 50.0):
         import torch
-        self.cv, self.torch, self.reps, self.slow_ms = cv, torch, reps, slow_ms
+        self.torch, self.reps, self.slow_ms = torch, reps, slow_ms
 
-    def run(self, kind, op, ty, arm, d, s, sd):
-        L = self.cv.lib
-        scr = self.cv.scratch(L.cvlhip_seg_scratch(sd.n, sd.m))
-        f = L.cvlhip_seg_scan if kind == "scan" else L.cvlhip_seg_reduce
-        f(OPS[op], 1 if ty == "d" else 0, arm, d, s, sd, sd.n, sd.m, scr)
-
-    def time_arm(self, kind, op, ty, arm, d, s, sd) -> float:
+    def __call__(self, fn) -> float:
         torch = self.torch
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         times = []
         for r in range(self.reps + 1):
             torch.cuda.synchronize()
             e0.record()
-            self.run(kind, op, ty, arm, d, s, sd)
+            fn()
             e1.record()
             e1.synchronize()
             t = e0.elapsed_time(e1)
@@ -110,11 +119,27 @@ class Autotuner:
                 return t
         return float(np.median(times))
 
+
+class Autotuner:
+    """Times every arm of a segmented op."""
+
+    def __init__(self, cv, reps: int = 5):
+        self.cv, self.timer = cv, Timer(reps)
+
+    def run(self, kind, op, ty, arm, d, s, sd, scr):
+        L = self.cv.lib
+        f = L.cvlhip_seg_scan if kind == "scan" else L.cvlhip_seg_reduce
+        f(OPS[op], 1 if ty == "d" else 0, arm, d, s, sd, sd.n, sd.m, scr)
+
     def tune(self, kind, op, ty, d, s, sd) -> np.ndarray:
-        return np.array([self.time_arm(kind, op, ty, a, d, s, sd) for a in range(len(ARMS))])
+        scr = self.cv.scratch(self.cv.lib.cvlhip_seg_scratch(sd.n, sd.m))
+        return np.array([self.timer(lambda a=a: self.run(kind, op, ty, a, d, s, sd, scr))
+                         for a in range(len(ARMS))])
 
 
 def features(cv, sd, kind: str, ty: str) -> np.ndarray:
+    """The network's input for this descriptor and kind of operation, as the
+    library computes it (from the histogram mke_fov stored)."""
     F = cv.lib.cvlhip_seg_features_count()
     out = np.zeros(F, np.float32)
     cv.lib.cvlhip_segd_features(sd, sd.n, sd.m, out)
@@ -182,6 +207,10 @@ class Dispatcher:
     def choose(self, feat: np.ndarray) -> int:
         return int(np.argmin(self.predict(feat[None, :])[0]))
 
+    def install(self):
+        """Make the library's mke_fov pick with these parameters."""
+        self.cv.lib.cvlhip_sched_set_params(self.params)
+
     def save(self, path: str = MODEL_PATH):
         np.savez(path, params=self.params.cpu().numpy(),
                  dims=np.array([self.F, self.H, self.A, self.P], np.int64))
@@ -195,49 +224,117 @@ class Dispatcher:
         return self
 
 
-# ------------------------------------------------------------------ data and evaluation
-def gather(cv, workloads, n: int, ops=(("scan", "add", "d"), ("reduce", "max", "d")), seed: int = 0,
-           log=print):
-    """(X, T, M, meta): features and log-times of every arm for every workload."""
-    import torch
-    tuner = Autotuner(cv)
+# ------------------------------------------------------------------ data
+class Inputs:
+    """Value vectors of the largest size, both types, from splitmix64."""
+
+    def __init__(self, cv, nmax: int):
+        import torch
+        self.xi = torch.empty(nmax, dtype=torch.int32, device=cv.device)
+        cv.lib.cvlhip_fill_splitmix_u32(self.xi, nmax, 0x4E45534C00000003)
+        self.xd = torch.empty(nmax, dtype=torch.float64, device=cv.device)
+        cv.lib.cvlhip_fill_splitmix_f64(self.xd, nmax, 0x4E45534C00000003, -1.0, 1.0)
+        self.out_i = torch.empty_like(self.xi)
+        self.out_d = torch.empty_like(self.xd)
+
+    def of(self, ty, n):
+        return (self.xd[:n], self.out_d[:n]) if ty == "d" else (self.xi[:n], self.out_i[:n])
+
+
+def gather(cv, dists, sizes, kinds=KINDS, seed: int = 0, log=print, reps: int = 5):
+    """(X, T, M, meta): features and log-times of every arm for every
+    (distribution, size, kind)."""
+    tuner = Autotuner(cv, reps)
     rng = np.random.default_rng(seed)
-    x = torch.empty(n, dtype=torch.float64, device=cv.device)
-    cv.lib.cvlhip_fill_splitmix_f64(x, n, 0x4E45534C00000003, -1.0, 1.0)
-    d = torch.empty_like(x)
+    inp = Inputs(cv, max(sizes))
     X, T, M, meta = [], [], [], []
-    for (kind, param) in workloads:
-        lens = lengths(kind, n, param, rng)
-        sd = cv.segd(lens)
-        for (k, op, ty) in ops:
-            t = tuner.tune(k, op, ty, d, x, sd)
-            X.append(features(cv, sd, k, ty))
-            T.append(np.log(t))
-            M.append(np.ones(len(ARMS), np.float32))
-            meta.append({"dist": kind, "param": param, "n": n, "m": int(lens.size), "op": f"{op}_{k}",
-                         "ms": [round(float(v), 4) for v in t]})
-            log(json.dumps(meta[-1]))
+    for (dist, param) in dists:
+        for n in sizes:
+            lens = lengths(dist, n, param, rng)
+            sd = cv.segd(lens)
+            for (k, op, ty) in kinds:
+                s, d = inp.of(ty, n)
+                t = tuner.tune(k, op, ty, d, s, sd)
+                X.append(features(cv, sd, k, ty))
+                T.append(np.log(t))
+                M.append(np.ones(len(ARMS), np.float32))
+                meta.append({"dist": dist, "param": param, "n": n, "m": int(lens.size), "op": f"{op}_{k}_{ty}",
+                             "ms": [round(float(v), 5) for v in t]})
+                log(json.dumps(meta[-1]))
     return (np.array(X, np.float32), np.array(T, np.float32), np.array(M, np.float32), meta)
 
 
-def evaluate(disp: Dispatcher, X, T, meta):
-    """Regret of the dispatcher against the autotuner, and totals per arm."""
-    pred = disp.predict(X)
+def offline_regret(disp: Dispatcher, X, T):
+    """Regret of the network's picks on measured arm times (no decision cost)."""
     times = np.exp(T)
-    pick = pred.argmin(axis=1)
-    best = times.min(axis=1)
+    pick = disp.predict(X).argmin(axis=1)
     chosen = times[np.arange(len(X)), pick]
-    per_arm_total = times.sum(axis=0)
+    return float(chosen.sum() / times.min(axis=1).sum() - 1.0)
+
+
+# ------------------------------------------------------------------ live evaluation
+def evaluate_live(cv, dists, sizes, kinds=KINDS, seed: int = 11, reps: int = 7, log=print):
+    """Times, per (distribution, size): every arm of every kind (the
+    autotuner's table), the CVL entry (add_sez, add_sed, max_rez, max_red)
+    on a descriptor built by mke_fov with the library's dispatcher on, and
+    the decision itself: mke_fov with the dispatcher on minus off (the
+    histogram and the network's forward pass), charged once per descriptor.
+    Returns the per-case records and the totals."""
+    import torch
+    from .cvl import Segd
+    L = cv.lib
+    timer = Timer(reps)
+    tuner = Autotuner(cv, reps)
+    rng = np.random.default_rng(seed)
+    inp = Inputs(cv, max(sizes))
+    cases, tot_arm, tot_learned, tot_best, tot_decision = [], np.zeros(len(ARMS)), 0.0, 0.0, 0.0
+    for (dist, param) in dists:
+        for n in sizes:
+            lens_h = lengths(dist, n, param, rng)
+            m = int(lens_h.size)
+            lens = torch.from_numpy(lens_h).to(cv.device)
+            units = L.siz_fos(n, m)
+            bufs = {on: torch.zeros(units * 8 // 4 + 64, dtype=torch.int32, device=cv.device) for on in (0, 1)}
+            bscr = torch.empty(L.mke_fov_scratch(n, m) * 8 + 256, dtype=torch.uint8, device=cv.device)
+            build_ms = {}
+            for on in (0, 1):
+                L.cvlhip_sched_enable(on)
+                build_ms[on] = timer(lambda on=on: L.mke_fov(bufs[on], lens, n, m, bscr))
+            L.cvlhip_sched_enable(1)
+            decision = max(0.0, build_ms[1] - build_ms[0])
+            sd = Segd(bufs[1], n, m)
+            rec = {"dist": dist, "param": param, "n": n, "m": m, "build_ms": build_ms[0],
+                   "decision_ms": round(decision, 5), "kinds": {}}
+            for kind in kinds:
+                k, op, ty = kind
+                s, d = inp.of(ty, n)
+                arms = tuner.tune(k, op, ty, d if k == "scan" else d[:m], s, sd)
+                name = CVL_NAME[kind]
+                scr = cv.scratch(getattr(L, name + "_scratch")(n, m))
+                dst = d if k == "scan" else d[:m]
+                entry = timer(lambda: getattr(L, name)(dst, s, sd, n, m, scr))
+                pick = L.cvlhip_segd_pick(sd, 1 if k == "reduce" else 0, 8 if ty == "d" else 4)
+                rec["kinds"][name] = {"arm_ms": [round(float(v), 5) for v in arms],
+                                      "best_arm": ARMS[int(arms.argmin())], "pick": ARMS[pick] if pick >= 0 else None,
+                                      "entry_ms": round(entry, 5)}
+                tot_arm += arms
+                tot_best += float(arms.min())
+                tot_learned += entry
+            tot_learned += decision
+            tot_decision += decision
+            cases.append(rec)
+            log(json.dumps(rec))
+    fixed = {ARMS[a]: float(tot_arm[a]) for a in range(len(ARMS))}
+    best_fixed = min(fixed, key=fixed.get)
     return {
-        "cases": len(X),
-        "learned_total_ms": float(chosen.sum()),
-        "autotuned_total_ms": float(best.sum()),
-        "regret_total": float(chosen.sum() / best.sum() - 1.0),
-        "max_case_regret": float((chosen / best - 1.0).max()),
-        "fixed_arm_total_ms": {ARMS[a]: float(per_arm_total[a]) for a in range(len(ARMS))},
-        "best_fixed_arm": ARMS[int(per_arm_total.argmin())],
-        "learned_beats_every_fixed_arm": bool(chosen.sum() < per_arm_total.min()),
-        "picks": [ARMS[a] for a in pick],
+        "cases": cases,
+        "learned_total_ms": tot_learned,
+        "decision_total_ms": tot_decision,
+        "autotuned_total_ms": tot_best,
+        "regret": tot_learned / tot_best - 1.0,
+        "fixed_arm_total_ms": fixed,
+        "best_fixed_arm": best_fixed,
+        "learned_vs_best_fixed": tot_learned / fixed[best_fixed] - 1.0,
     }
 
 
@@ -245,21 +342,33 @@ def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     from .cvl import Cvl
     cv = Cvl()
-    n_train = 1 << 24
-    n_eval = 1 << 26
     t0 = time.time()
-    X, T, M, meta = gather(cv, TRAIN_WORKLOADS, n_train)
-    # a subset again at the evaluation size, with the steep Zipf tails whose
-    # very short segments favour other tiles than the rest
-    X2, T2, M2, meta2 = gather(cv, TRAIN_WORKLOADS[::3] + [("zipf", 2.2), ("zipf", 3.0)], n_eval, seed=5)
-    Xtr = np.concatenate([X, X2]); Ttr = np.concatenate([T, T2]); Mtr = np.concatenate([M, M2])
-    disp = Dispatcher(cv)
-    losses = disp.train(Xtr, Ttr, Mtr, epochs=3000)
+    X, T, M, meta = gather(cv, TRAIN_DISTS, TRAIN_SIZES, seed=0)
+    Xv, Tv, Mv, metav = gather(cv, VALID_DISTS, VALID_SIZES, seed=5)
+    # the training run: the seed and length with the lowest validation regret
+    best = None
+    for seed in (0, 1, 2):
+        disp = Dispatcher(cv, seed=seed)
+        done = 0
+        for epochs in (1500, 1500, 3000):
+            losses = disp.train(X, T, M, epochs=epochs, seed=seed + done)
+            done += epochs
+            r = offline_regret(disp, Xv, Tv)
+            print(json.dumps({"seed": seed, "epochs": done, "loss": float(np.mean(losses[-50:])),
+                              "valid_regret": r}), flush=True)
+            if best is None or r < best[0]:
+                best = (r, seed, done, disp.params.clone())
+    disp = Dispatcher(cv, seed=best[1])
+    disp.params = best[3]
     disp.save(argv[2] if len(argv) > 2 else MODEL_PATH)
-    Xe, Te, Me, metae = gather(cv, EVAL_WORKLOADS, n_eval, seed=11)
-    res = {"train_cases": int(len(Xtr)), "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),
-           "eval": evaluate(disp, Xe, Te, metae), "eval_cases": metae, "seconds": round(time.time() - t0, 1)}
-    print(json.dumps(res), flush=True)
+    disp.install()
+    res = {"train_cases": int(len(X)), "valid_cases": int(len(Xv)), "valid_regret": best[0],
+           "chosen_seed": best[1], "chosen_epochs": best[2],
+           "train_offline_regret": offline_regret(disp, X, T),
+           "eval": evaluate_live(cv, EVAL_DISTS, EVAL_SIZES), "seconds": round(time.time() - t0, 1),
+           "train_timings": meta, "valid_timings": metav}
+    print(json.dumps({k: v for k, v in res.items() if k not in ("train_timings", "valid_timings")})[:4000],
+          flush=True)
     out = argv[1] if len(argv) > 1 else None
     if out:
         with open(out, "w") as f:

@@ -75,6 +75,8 @@ def _expr(op: S.SuperOp, e) -> str:
         return _expr(op, e.args[0])
     if e.op == "GATHER":
         src, seg = e.args[0].index, e.args[2].index
+        if len(e.args) == 4:                  # segmented: the segment of element ix in the destination
+            return f"gather_seg(s{src}, v{seg}, v{e.args[3].index}, {_expr(op, e.args[1])}, ix)"
         return f"gather(s{src}, n{seg}, {_expr(op, e.args[1])})"
     if e.op == "SELECT":
         f, a, b = (_expr(op, x) for x in e.args)
@@ -107,15 +109,16 @@ _REDUCE_OP = {("+_REDUCE", "INT"): "AddZ", ("+_REDUCE", "FLOAT"): "AddD", ("MAX_
 
 def _reduce_op(o) -> list:
     """A super-op whose root reduces an expression over the segments of a
-    descriptor: the CVL's own segmented-reduce kernel (scan_kernel.hpp,
-    16-row tiles, the same association as *_re*) with an input loader that
-    evaluates the expression, so the per-element values are never stored."""
+    descriptor: the CVL's own segmented reduction (segsched.hpp,
+    seg_reduce_run: the schedule the descriptor names, the same association
+    as *_re*) with an input loader that evaluates the expression, so the
+    per-element values are never stored."""
     k, (rop, r) = o.number, o.reduce
     op = _REDUCE_OP[(rop, o.result)]
     rt = CTYPE[o.result]
     used = sorted(_used_args(o.body, set()))
     gsrc = S.gather_sources(o.body)
-    gseg = sorted(set(gsrc.values()))
+    gseg = sorted(set(gsrc.values()) | S.gather_dests(o.body))
     fields = [f"const {CTYPE[o.params[i]]} *s{i};" for i in used]
     fields += [f"const {CTYPE[o.params[i]]} *s{i};" for i in sorted(gsrc) if i not in used]
     fields += [f"const void *sd{j};" for j in gseg]
@@ -125,6 +128,9 @@ def _reduce_op(o) -> list:
           f"    __device__ __forceinline__ void load(int64_t base, int64_t n, bool aligned, {rt} (&x)[V]) const {{"]
     for j in gseg:
         L.append(f"        const long long n{j} = *(const long long *)sd{j};")
+        L.append(f"        const cvlhip::SegdView v{j} = cvlhip::SegdView::from_header(sd{j});")
+        L.append(f"        (void)n{j};")
+        L.append(f"        (void)v{j};")
     # Whole rows of aligned arguments: one vector load of V elements per
     # streamed argument (`aligned` is set by the host when every streamed
     # argument starts at a multiple of its vector size); gathered sources
@@ -133,12 +139,13 @@ def _reduce_op(o) -> list:
     for i in used:
         L.append(f"            const typename FusedVec<{CTYPE[o.params[i]]}, V>::type w{i} = "
                  f"*(const typename FusedVec<{CTYPE[o.params[i]]}, V>::type *)(s{i} + base);")
-    L += ["#pragma unroll", "            for (int k = 0; k < V; k++) {"]
+    L += ["#pragma unroll", "            for (int k = 0; k < V; k++) {",
+          "                const long long ix = base + k;", "                (void)ix;"]
     for i in used:
         L.append(f"                const {CTYPE[o.params[i]]} a{i} = w{i}[k];")
     L += [f"                x[k] = ({rt})({_expr(o, o.body)});", "            }", "            return;", "        }"]
     L += ["#pragma unroll", "        for (int k = 0; k < V; k++) {", "            const int64_t i = base + k;",
-          "            if (i < n) {"]
+          "            const long long ix = i;", "            (void)ix;", "            if (i < n) {"]
     for i in used:
         L.append(f"                const {CTYPE[o.params[i]]} a{i} = s{i}[i];")
     L += [f"                x[k] = ({rt})({_expr(o, o.body)});",
@@ -149,7 +156,7 @@ def _reduce_op(o) -> list:
     init += [f"(const void *)s{j}" for j in gseg]
     L += [f'extern "C" void fused{k}({", ".join(hparams)}) {{',
           f"    using Op = cvlhip::{op};",
-          "    using Sh = cvlhip::ScanShape<Op, cvlhip::kSegReduceRows, cvlhip::kSegReduceBlock>;",
+          "    constexpr int V = cvlhip::lane_values<Op>::value;",
           "    hipStream_t st = (hipStream_t)cvlhip_get_stream();"]
     for i in range(len(o.params)):
         if i not in used and i not in gsrc and i not in gseg and i != r:
@@ -158,28 +165,18 @@ def _reduce_op(o) -> list:
           f"    cvlhip::k_empty_identity_hdr<{rt}><<<256, 256, 0, st>>>(({rt} *)d, sd, Op::cvl_id());",
           f'    fused_check("fused{k}");',
           "    if (len <= 0) return;",
-          "    const long long tiles = (len + Sh::TILE - 1) / Sh::TILE;",
-          "    void *recs = (char *)scratch + 256;",
-          "    constexpr int V = Sh::V;",
           "    const int aligned = " + (" && ".join(
               f"((unsigned long long)s{i} % (sizeof({CTYPE[o.params[i]]}) * V)) == 0" for i in used) or "0") + ";",
-          "    // the CVL's own two-step segmented reduction (same tiles, same bits)",
-          f"    cvlhip::k_scan<Op, cvlhip::kSegReduceRows, true, true, InFused{k}, cvlhip::kSegReduceBlock, true>",
-          "        <<<(unsigned)tiles, cvlhip::kSegReduceBlock, 0, st>>>(",
-          f"        InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len, cvlhip::SegBits{{nullptr, nullptr, nullptr, sd}},",
-          "        (uint64_t *)recs, nullptr, aligned, (unsigned *)cvlhip_device_error_word());",
+          "    // the CVL's own segmented reduction, under the schedule the descriptor",
+          "    // names (the same tiles and bits as the CVL call on the same descriptor)",
+          f"    cvlhip::seg_reduce_run<Op, InFused{k}>(InFused{k}{{{', '.join(init)}}}, ({rt} *)d, (int64_t)len,",
+          "                                          cvlhip::SegdView(), sd, true, scratch, aligned != 0,",
+          "                                          (unsigned *)cvlhip_device_error_word(), -1, st);",
           f'    fused_check("fused{k}");',
-          "    if (tiles > 1) {",
-          f"        cvlhip::k_seg_reduce_fix<Op><<<1, cvlhip::kFixThreads, 0, st>>>(cvlhip::NolbRecords<{rt}>::at(recs, tiles),",
-          f"                                                                       tiles, ({rt} *)d);",
-          f'        fused_check("fused{k}");',
-          "    }",
           "}",
           f'extern "C" int fused{k}_scratch(int n, int m) {{',
           "    (void)m;",
-          f"    using Sh = cvlhip::ScanShape<cvlhip::{op}, cvlhip::kSegReduceRows, cvlhip::kSegReduceBlock>;",
-          "    const long long tiles = ((long long)n + Sh::TILE - 1) / Sh::TILE;",
-          f"    return (int)((256 + cvlhip::NolbRecords<{rt}>::bytes(tiles) + 256 + 7) / 8);",
+          f"    return (int)((cvlhip::sched_scratch_bytes<cvlhip::{op}, true>(n) + 7) / 8);",
           "}",
           f'extern "C" unsigned fused{k}_inplace(void) {{ return 0u; }}', ""]
     return L
@@ -192,7 +189,7 @@ def generate(ops, library_name: str = "fused") -> str:
     out += [f"//   {o.text}" for o in ops]
     out += ['#include <hip/hip_runtime.h>', '#include <stdio.h>', '#include <stdlib.h>',
             '#include "cvl.h"', '#include "cvl_hip_ext.h"', '#include "vcode_fused.h"',
-            '#include "elwise_ops.hpp"', '#include "scan_kernel.hpp"', "",
+            '#include "elwise_ops.hpp"', '#include "segsched.hpp"', "",
             "static int fused_grid(long long n) {",
             "    long long g = (n + 1023) / 1024;",
             "    return g > 16384 ? 16384 : (g < 1 ? 1 : (int)g);",
@@ -202,6 +199,15 @@ def generate(ops, library_name: str = "fused") -> str:
             "template <class T>",
             "__device__ __forceinline__ T gather(const T *src, long long n, int i) {",
             "    return i >= 0 && (long long)i < n ? src[i] : (T)0;",
+            "}", "",
+            "// Segmented back-permute of element k: its segment j in the destination",
+            "// descriptor vd, index i within segment j of the source descriptor vs.",
+            "template <class T>",
+            "__device__ __forceinline__ T gather_seg(const T *src, const cvlhip::SegdView &vs,",
+            "                                        const cvlhip::SegdView &vd, int i, long long k) {",
+            "    if (vd.m == 1) return i >= 0 && (long long)i < vs.n ? src[i] : (T)0;",
+            "    const int j = vd.seg_of(k);",
+            "    return i >= 0 && i < vs.lens[j] ? src[vs.offs[j] + i] : (T)0;",
             "}", "",
             "static void fused_check(const char *what) {",
             "    hipError_t e = hipGetLastError();",
@@ -217,18 +223,24 @@ def generate(ops, library_name: str = "fused") -> str:
         k = o.number
         used = sorted(_used_args(o.body, set()))
         gsrc = S.gather_sources(o.body)
+        gdesc = sorted(set(gsrc.values()) | S.gather_dests(o.body))
         rt = CTYPE[o.result]
         params = [f"{rt} *d"]                  # may alias argument 0 (in place)
         params += [f"const {CTYPE[o.params[i]]} *s{i}" for i in used]
         params += [f"const {CTYPE[o.params[i]]} *s{i}" for i in sorted(gsrc) if i not in used]
-        params += [f"const int *sd{j}" for j in sorted(set(gsrc.values()))]
+        params += [f"const int *sd{j}" for j in gdesc]
         params.append("long long n")
         out.append(f"__global__ void __launch_bounds__(256) k_fused{k}({', '.join(params)}) {{")
         out.append("    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;")
         out.append("    const long long stride = (long long)gridDim.x * blockDim.x;")
-        for j in sorted(set(gsrc.values())):      # element count from the descriptor's header
+        for j in gdesc:                           # shape from the descriptor's header
             out.append(f"    const long long n{j} = *(const long long *)sd{j};")
+            out.append(f"    const cvlhip::SegdView v{j} = cvlhip::SegdView::from_header(sd{j});")
+            out.append(f"    (void)n{j};")
+            out.append(f"    (void)v{j};")
         out.append("    for (; i < n; i += stride) {")
+        out.append("        const long long ix = i;")
+        out.append("        (void)ix;")
         for i in used:
             out.append(f"        const {CTYPE[o.params[i]]} a{i} = s{i}[i];")
         out.append(f"        d[i] = ({rt})({_expr(o, o.body)});")
@@ -237,20 +249,21 @@ def generate(ops, library_name: str = "fused") -> str:
         hparams = ["vec_p d"] + [f"vec_p s{i}" for i in range(len(o.params))] + ["int len", "vec_p scratch"]
         out.append(f'extern "C" void fused{k}({", ".join(hparams)}) {{')
         out.append("    (void)scratch;")
-        passed = set(used) | set(gsrc) | set(gsrc.values())
+        passed = set(used) | set(gsrc) | set(gdesc)
         for i in range(len(o.params)):
             if i not in passed:
                 out.append(f"    (void)s{i};")
         out.append("    if (len <= 0) return;")
         args = [f"({rt} *)d"] + [f"(const {CTYPE[o.params[i]]} *)s{i}" for i in used]
         args += [f"(const {CTYPE[o.params[i]]} *)s{i}" for i in sorted(gsrc) if i not in used]
-        args += [f"(const int *)s{j}" for j in sorted(set(gsrc.values()))]
+        args += [f"(const int *)s{j}" for j in gdesc]
         args.append("(long long)len")
         out.append(f"    k_fused{k}<<<fused_grid(len), 256, 0, (hipStream_t)cvlhip_get_stream()>>>({', '.join(args)});")
         out.append(f'    fused_check("fused{k}");')
         out.append("}")
         out.append(f'extern "C" int fused{k}_scratch(int len) {{ (void)len; return 0; }}')
-        inplace = 1 if o.params and o.params[0] != "SEGDES" else 0
+        # argument 0 may be overwritten unless a gather reads it elsewhere
+        inplace = 1 if o.params and o.params[0] != "SEGDES" and 0 not in gsrc else 0
         out.append(f'extern "C" unsigned fused{k}_inplace(void) {{ return {inplace}u; }}')
         out.append("")
     # descriptors
@@ -318,8 +331,9 @@ def build(defs_text: str, out_so: str, hip_path: str | None = None, verbose: boo
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc"),
            hip_path, "-o", out_so + ".tmp",
-           # the CVL library supplies the stream; found next to the fused/ directory
-           "-L", PKG, "-lcvl_hip", "-Wl,-rpath,$ORIGIN/.."]
+           # the CVL library supplies the stream; found relative to the library's directory
+           "-L", PKG, "-lcvl_hip",
+           "-Wl,-rpath,$ORIGIN/" + os.path.relpath(PKG, os.path.dirname(os.path.abspath(out_so)))]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stderr)

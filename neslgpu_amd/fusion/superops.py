@@ -21,6 +21,10 @@ them.  And `(GATHER TYPE @ $src e $s)` reads $src at the index e evaluates to
 (an unsegmented BPERMUTE folded into the fused op; $s is the one-segment
 descriptor of $src, whose length bounds the index: an index outside
 [0, length) gives 0).  $src need not have the length of the other arguments.
+The four-argument form `(GATHER TYPE @ $src e $ss $sd)` is a whole segmented
+BPERMUTE (ref nesl/cvl/serial/vprims.c:416-467): element k of the result,
+in segment j of $sd, reads $src at e plus the start of segment j of $ss (an
+index outside [0, length of segment j of $ss) gives 0).
 """
 from __future__ import annotations
 
@@ -166,10 +170,11 @@ def typecheck(op: SuperOp) -> str:
                 raise DefinitionError("only constants can be distributed inside a fused op")
             return ty(e.args[0])
         if e.op == "GATHER":
-            if len(e.args) != 3 or not isinstance(e.args[0], Arg) or not isinstance(e.args[2], Arg):
-                raise DefinitionError("GATHER needs $src, an index and $s")
-            if op.params[e.args[2].index] != "SEGDES" or ty(e.args[1]) != "INT":
-                raise DefinitionError("GATHER index must be INT and $s a segment descriptor")
+            if len(e.args) not in (3, 4) or not isinstance(e.args[0], Arg) or \
+                    not all(isinstance(x, Arg) for x in e.args[2:]):
+                raise DefinitionError("GATHER needs $src, an index and one or two descriptors")
+            if any(op.params[x.index] != "SEGDES" for x in e.args[2:]) or ty(e.args[1]) != "INT":
+                raise DefinitionError("GATHER index must be INT and its descriptors segment descriptors")
             t = ty(e.args[0])
             if e.type and e.type != t:
                 raise DefinitionError("GATHER type")
@@ -200,13 +205,25 @@ def typecheck(op: SuperOp) -> str:
 
 
 def gather_sources(e, acc=None):
-    """{src index: segd index} of the GATHER forms of an expression."""
+    """{src index: segd index} of the GATHER forms of an expression (the
+    descriptor of the source)."""
     acc = {} if acc is None else acc
     if isinstance(e, Apply):
         if e.op == "GATHER":
             acc[e.args[0].index] = e.args[2].index
         for x in e.args:
             gather_sources(x, acc)
+    return acc
+
+
+def gather_dests(e, acc=None):
+    """Indices of the destination descriptors of four-argument GATHERs."""
+    acc = set() if acc is None else acc
+    if isinstance(e, Apply):
+        if e.op == "GATHER" and len(e.args) == 4:
+            acc.add(e.args[3].index)
+        for x in e.args:
+            gather_dests(x, acc)
     return acc
 
 

@@ -77,6 +77,9 @@ VARIANTS = [
     ("qs.1048576.full", {"base": "qs.1000.vcode", "full": True, "size": 1 << 20,
                          "key_range": 2147483647, "digest": True}),
     ("quickhull.100000.full", {"base": "quickhull.1000.vcode", "full": True, "size": 100000, "digest": True}),
+    # the benchmark's size: 2^24 points in the unit disk, the whole hull
+    ("quickhull.16777216.full", {"base": "quickhull.1000.vcode", "full": True, "size": 1 << 24,
+                                 "digest": True}),
     # fused programs print what their unfused forms print
     ("qs.1000.fcode.full", {"base": "qs.1000.fcode", "full": True, "same_as": "qs.1000.full",
                             "superops": "qs.1000.superops"}),
@@ -93,6 +96,34 @@ OWN_VARIANTS = [
 ]
 
 ARENA_UNITS = "200000000"
+
+# SpMV at the benchmark's size (2^22 rows, power-law lengths with mean 16
+# and maximum 2^16, 2^26 nonzeros), one product, y printed: the serial CVL's
+# y at a sample of rows (every 1024th row and the 64 longest rows) is kept
+# as a fixture; the whole y is checked against an extended-precision sum.
+SPMV_SAMPLE = os.path.join(OUT, "spmv.4194304.sample.npz")
+
+
+def spmv_full_sample():
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    from neslgpu_amd import workloads as W
+    rows = W.spmv_rows()
+    text = W.spmv_y_program(open(os.path.join(OWN, "spmv.vcode")).read())
+    with tempfile.NamedTemporaryFile("w", suffix=".vcode", delete=False) as f:
+        f.write(text)
+    r = subprocess.run([VINTERP, "-B", SERIAL, "-m", str((1 << 31) - 1), f.name], capture_output=True, text=True,
+                       timeout=1200, input=W.spmv_stdin(rows, W.SPMV_ROWS))
+    os.unlink(f.name)
+    if r.returncode != 0:
+        raise SystemExit(f"spmv: exit {r.returncode}\n{r.stderr[-2000:]}")
+    lines = r.stdout.splitlines()
+    y = np.array(lines[-2].strip("() ").split(), dtype=np.float64)
+    assert y.size == rows.size, (y.size, rows.size)
+    idx = np.union1d(np.arange(0, rows.size, 1024), np.argsort(rows, kind="stable")[-64:]).astype(np.int64)
+    np.savez(SPMV_SAMPLE, rows_index=idx, y=y[idx], row_len=rows[idx],
+             rows_sha256=np.array(hashlib.sha256(rows.tobytes()).hexdigest()))
+    print("spmv full-size sample", idx.size, "rows", flush=True)
 
 
 def variant_text(spec):
@@ -213,8 +244,13 @@ def main():
             size = int(V.entry_size(V.read(os.path.join(PROGS, base))))
             manifest[name] = dict(manifest[base], expected=base, superops=name[: -len(".fcode")] + ".superops",
                                   base=name, size=size)
+    for name, _ in PROGRAMS:
+        manifest.setdefault(name, {})["sha256_of_program"] = hashlib.sha256(
+            open(os.path.join(PROGS, name), "rb").read()).hexdigest()
+        manifest[name]["source"] = dict(PROGRAMS)[name]
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
+    spmv_full_sample()
 
 
 if __name__ == "__main__":

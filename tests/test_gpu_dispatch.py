@@ -2,7 +2,11 @@
 
 Every schedule ("arm") of cvlhip_seg_scan / cvlhip_seg_reduce matches the
 oracle: bit-exact for integer and min/max operators, within the compensated
-tolerance for double +.  The network's HIP gradients match the numpy
+tolerance for double +.  The CVL entries run the arm mke_fov picked for the
+descriptor: a network forced to pick each arm in turn leaves that pick in
+the descriptor and the entries' results still match the oracle; the pick
+the library computes on the device equals the argmin of the HIP forward
+pass on the same features.  The network's HIP gradients match the numpy
 restatement (itself pinned by finite differences), and training on timings
 lowers the loss."""
 import numpy as np
@@ -22,6 +26,7 @@ def cv():
 
 
 DISTS = [("geometric", 3), ("geometric", 300), ("zipf", 1.5), ("const", 1), ("bimodal", 0)]
+SENTINEL = {"z": 0x5A5A5A5A, "d": float("nan")}     # what an unwritten output element holds
 
 
 @pytest.mark.parametrize("dist", DISTS)
@@ -48,14 +53,14 @@ def test_every_arm_matches_oracle(cv, dist):
             want_red = O.segreduce(op, ty, x, lens)
             for arm in range(len(D.ARMS)):
                 print("arm", D.ARMS[arm], dist, ty, op, flush=True)
-                d = torch.empty_like(dx)
+                d = torch.full_like(dx, SENTINEL[ty])
                 cv.lib.cvlhip_seg_scan(D.OPS[op], 1 if ty == "d" else 0, arm, d, dx, sd, n, m, scr)
                 got = host(d)
                 if ty == "d" and op == "add":
                     assert np.all(np.abs(got - ref_s) <= tol_s), (D.ARMS[arm], op, ty)
                 else:
                     eq(got, want_scan, f"scan {D.ARMS[arm]} {op}{ty} {dist}")
-                r = torch.empty(m, dtype=dx.dtype, device=cv.device)
+                r = torch.full((m,), SENTINEL[ty], dtype=dx.dtype, device=cv.device)
                 cv.lib.cvlhip_seg_reduce(D.OPS[op], 1 if ty == "d" else 0, arm, r, dx, sd, n, m, scr)
                 got = host(r)
                 if ty == "d" and op == "add":
@@ -112,3 +117,84 @@ def test_training_lowers_loss(cv):
     M = np.ones_like(T)
     losses = disp.train(X, T, M, epochs=600, lr=3e-3, batch=64)
     assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:20])
+
+
+def forcing_params(dims, arm):
+    """A network whose prediction is lowest for `arm` whatever the input:
+    every weight 0, the output biases 0 except -1 at `arm`."""
+    F, H, A, P = dims
+    p = np.zeros(P, np.float32)
+    p[P - A + arm] = -1.0
+    return p
+
+
+def test_forced_picks_run_through_cvl_entries(cv):
+    """Each arm, picked by mke_fov, is what add_sez / add_sed / max_rez /
+    min_red / mul_rez run; results match the oracle."""
+    import torch
+    from neslgpu_amd import dispatch as D
+    L = cv.lib
+    dims = D.mlp_dims(L)
+    saved = torch.empty(dims[3], dtype=torch.float32, device=cv.device)
+    L.cvlhip_sched_get_params(saved)
+    had = L.cvlhip_sched_has_model()
+    rng = np.random.default_rng(21)
+    n = 3 * L.cvlhip_sched_min_n() + 77
+    lens = D.lengths("zipf", n, 1.6, rng, lmax=1 << 14)
+    lens = np.insert(lens, [1, 5], 0)
+    m = lens.size
+    xi = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
+    xd = rng.uniform(-1, 1, n)
+    dxi, dxd = torch.from_numpy(xi).to(cv.device), torch.from_numpy(xd).to(cv.device)
+    ref_s, absp = O.segscan_add_neumaier(xd, lens)
+    tol_s = fp_add_tol_vec(absp, seg_lengths_per_element(lens))
+    try:
+        for arm in range(len(D.ARMS)):
+            L.cvlhip_sched_set_params(torch.from_numpy(forcing_params(dims, arm)).to(cv.device))
+            sd = cv.segd(lens)
+            for red in (0, 1):
+                for eb in (4, 8):
+                    assert L.cvlhip_segd_pick(sd, red, eb) == arm, (arm, red, eb)
+            print("forced", D.ARMS[arm], n, m, flush=True)
+            d = torch.full_like(dxi, SENTINEL["z"])
+            L.add_sez(d, dxi, sd, n, m, cv.scratch(L.add_sez_scratch(n, m)))
+            eq(host(d), O.segscan("add", "z", xi, lens), f"add_sez {D.ARMS[arm]}")
+            d = torch.full_like(dxd, SENTINEL["d"])
+            L.add_sed(d, dxd, sd, n, m, cv.scratch(L.add_sed_scratch(n, m)))
+            assert np.all(np.abs(host(d) - ref_s) <= tol_s), ("add_sed", D.ARMS[arm])
+            for name, x, dx, ty in (("max_rez", xi, dxi, "z"), ("mul_rez", xi, dxi, "z"), ("min_red", xd, dxd, "d")):
+                r = torch.full((m,), SENTINEL[ty], dtype=dx.dtype, device=cv.device)
+                getattr(L, name)(r, dx, sd, n, m, cv.scratch(getattr(L, name + "_scratch")(n, m)))
+                eq(host(r), O.segreduce(name[:3], ty, x, lens), f"{name} {D.ARMS[arm]}")
+    finally:
+        L.cvlhip_sched_set_params(saved if had else None)
+    assert L.cvlhip_check_device_errors() == 0
+
+
+def test_small_descriptors_carry_no_pick(cv):
+    L = cv.lib
+    n = L.cvlhip_sched_min_n() - 1
+    sd = cv.segd(np.array([n // 2, n - n // 2], np.int32))
+    assert L.cvlhip_segd_pick(sd, 0, 4) == -1
+
+
+def test_device_pick_is_the_networks_argmin(cv):
+    """mke_fov's pick equals the argmin of cvlhip_mlp_forward on the
+    features cvlhip_segd_features reports, with the library's parameters."""
+    import torch
+    from neslgpu_amd import dispatch as D
+    L = cv.lib
+    if not L.cvlhip_sched_has_model():
+        pytest.skip("library built without a dispatcher model")
+    disp = D.Dispatcher(cv)
+    L.cvlhip_sched_get_params(disp.params)
+    rng = np.random.default_rng(5)
+    for dist, param in (("zipf", 1.5), ("geometric", 4), ("const", 1000), ("uniform", 2000)):
+        for log2n in (15, 18, 21):
+            lens = D.lengths(dist, 1 << log2n, param, rng)
+            sd = cv.segd(lens)
+            for kind, ty in (("scan", "z"), ("scan", "d"), ("reduce", "z"), ("reduce", "d")):
+                f = D.features(cv, sd, kind, ty)
+                want = disp.choose(f)
+                got = L.cvlhip_segd_pick(sd, 1 if kind == "reduce" else 0, 8 if ty == "d" else 4)
+                assert got == want, (dist, param, log2n, kind, ty, got, want)

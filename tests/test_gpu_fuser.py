@@ -1,0 +1,35 @@
+"""Programs compiled by this package's fuser (neslgpu_amd/fusion/fuser.py)
+run on the GPU: the .fcode the build wrote for each reference program
+(neslgpu_amd/fused/gen/) with the HIP library generated from its super-op
+definitions must print what the unfused program printed on the reference
+serial CVL (the goldens of tests/golden/vcode)."""
+import json
+import os
+
+import pytest
+
+from neslgpu_amd import vinterp as V
+from neslgpu_amd.build import generated
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "vcode")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def stable(text, volatile):
+    return [l for i, l in enumerate(text.splitlines()) if i not in volatile]
+
+
+@pytest.mark.parametrize("name", sorted(k for k in MANIFEST if k.endswith(".vcode")))
+def test_fuser_output_runs_on_the_gpu(name):
+    base = name[: -len(".vcode")]
+    fcode, lib = generated(base, "fcode"), generated(base, "so")
+    assert os.path.exists(fcode) and os.path.exists(lib), f"{fcode} not built (python -m neslgpu_amd.build)"
+    print("fuser output", base, flush=True)
+    rec = MANIFEST[name]
+    r = V.run(open(fcode).read(), fused=lib, timeout=240)
+    want = open(os.path.join(GOLD, name + ".out")).read()
+    assert stable(r.stdout, rec["volatile_stdout_lines"]) == stable(want, rec["volatile_stdout_lines"])
+    assert r.stderr == open(os.path.join(GOLD, name + ".err")).read()

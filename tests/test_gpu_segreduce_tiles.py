@@ -9,7 +9,9 @@ tiles, put empty segments at tile boundaries and pack thousands of
 one-element segments around them.  Results must equal the C oracle's
 (nesl/cvl/serial/vprims.c:128-201) exactly for integers and min/max, and be
 within the compensated-sum bound for double +, through the CVL entry
-points (8-row tiles) and both tile schedules of cvlhip_seg_reduce."""
+points and every schedule of cvlhip_seg_reduce (segsched.hpp: flat tiles
+of 8, 2 and 1 rows per wave, thread, wave and workgroup per segment), with
+the boundaries placed around the tiles of each flat schedule."""
 import numpy as np
 import pytest
 
@@ -19,7 +21,8 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 # elements per tile of a segmented reduction: 512 threads x rows x 16 bytes
-TILE8 = {"z": 8 * 8 * 64 * 4, "d": 8 * 8 * 64 * 2}
+def tile(ty, rows):
+    return 8 * rows * 64 * (4 if ty == "z" else 2)
 
 
 @pytest.fixture(scope="module")
@@ -46,13 +49,14 @@ def values(ty, n, rng):
     return rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
 
 
+@pytest.mark.parametrize("rows", [8, 2, 1])
 @pytest.mark.parametrize("ty", ["z", "d"])
-def test_segmented_reductions_at_tile_boundaries(cv, ty):
+def test_segmented_reductions_at_tile_boundaries(cv, ty, rows):
     import torch
     from neslgpu_amd import dispatch as D
-    rng = np.random.default_rng(77 if ty == "z" else 78)
+    rng = np.random.default_rng((77 if ty == "z" else 78) + rows)
     L = cv.lib
-    for name, lens in patterns(TILE8[ty], rng).items():
+    for name, lens in patterns(tile(ty, rows), rng).items():
         lens = np.asarray(lens, np.int32)
         n, m = int(lens.sum()), lens.size
         x = values(ty, n, rng)
@@ -66,12 +70,13 @@ def test_segmented_reductions_at_tile_boundaries(cv, ty):
                 want = O.segreduce(op, ty, x, lens)
             runs = [("entry", lambda: getattr(L, f"{op}_re{ty}")(
                 out, dx, sd, n, m, cv.scratch(getattr(L, f"{op}_re{ty}_scratch")(n, m))))]
-            for arm in (0, 1):       # 16-row and 8-row tiles
-                runs.append((f"arm{arm}", lambda arm=arm: L.cvlhip_seg_reduce(
+            for arm in range(len(D.ARMS)):
+                runs.append((D.ARMS[arm], lambda arm=arm: L.cvlhip_seg_reduce(
                     D.OPS[op], 1 if ty == "d" else 0, arm, out, dx, sd, n, m,
                     cv.scratch(L.cvlhip_seg_scratch(n, m)))))
             for how, run in runs:
-                print(name, op, ty, how, n, m, flush=True)
+                print(name, rows, op, ty, how, n, m, flush=True)
+                out.fill_(float("nan") if ty == "d" else 0x5A5A5A5A)   # a segment left unwritten shows
                 run()
                 got = host(out)
                 if ty == "d" and op == "add":

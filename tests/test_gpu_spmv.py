@@ -1,6 +1,7 @@
 """Nested SpMV (neslgpu_amd/vcode/programs/spmv.vcode) on the GPU, unfused,
-fused (FUSED 0 gathers x by column and multiplies by the values in one
-kernel) and with basic blocks replayed as HIP graphs (-G).
+fused by this package's fuser (neslgpu_amd/fusion/fuser.py: the gather of x
+by column, the product with the values and the segmented sum over the rows
+in one generated kernel) and with basic blocks replayed as HIP graphs (-G).
 
 The serial reference CVL ran the same program (golden spmv.small.full):
 row lengths, column indices, values and x are bit-exact; each y_i is a sum
@@ -18,7 +19,7 @@ import numpy as np
 import pytest
 
 from neslgpu_amd import vinterp as V
-from neslgpu_amd.build import FUSED_DIR
+from neslgpu_amd.build import generated
 
 pytestmark = pytest.mark.gpu
 
@@ -36,13 +37,13 @@ def parse_vec(line):
 
 
 def program(fused, k):
-    text = V.read(os.path.join(PROG, "spmv.fcode" if fused else "spmv.vcode"))
+    text = V.read(generated("spmv", "fcode") if fused else os.path.join(PROG, "spmv.vcode"))
     text = V.set_function_int_const(text, "MAIN", 10, k)
     return V.full_result(text)
 
 
 def run(fused, k, graphs):
-    lib = os.path.join(FUSED_DIR, "spmv.so") if fused else None
+    lib = generated("spmv", "so") if fused else None
     r = V.run(program(fused, k), fused=lib, graphs=graphs, stdin=REC["stdin"], timeout=120, timed=graphs)
     return r.stdout.splitlines()[:-1], r.stderr
 
