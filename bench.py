@@ -440,6 +440,10 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -455,9 +459,11 @@ def main():
         names = [x for x in names + ["cpy_wuz"] if x in a.only.split(",")]
     sw = Sweep(a.log2n, a.rank_log2n, names)
     byts = sw.bytes()
+    progress("inputs ready")
     for _ in range(a.warmup):
         sw.step(names)
     torch.cuda.synchronize()
+    progress("warm-up done")
     t0 = time.perf_counter()
     ms = time_gpu(lambda: sw.step(names), a.steps, torch)
     wall = (time.perf_counter() - t0) / a.steps * 1e3
@@ -468,6 +474,7 @@ def main():
         gbs = byts[k] / t / 1e6
         prims[k] = {"ms": round(t, 4), "GB/s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                     "bytes": byts[k]}
+        progress(f"{k} {prims[k]['ms']} ms")
     dom = max(prims, key=lambda k: prims[k]["ms"])
     extra = {}
     if "pk2_luz" in names:
@@ -521,10 +528,12 @@ def main():
     }
     if not a.no_irregular and not a.only:
         out["irregular_f64"] = irregular(a.irregular_log2n, max(3, a.steps))
+        progress("irregular done")
     if a.programs:
         out["programs"] = programs(3, tuple(a.programs.split(",")))
     elif not a.no_programs and not a.only:
         out["programs"] = programs(3)
+    progress("programs done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         if not a.no_programs and not a.only and not a.programs:

@@ -48,6 +48,13 @@ constexpr int blk_log() { return sizeof(T) == 4 ? kBlkLog4 : kBlkLog8; }
 #ifndef CVLHIP_PART_THREADS
 #define CVLHIP_PART_THREADS 1024
 #endif
+// 1: the first pass chains each tile's slots in the children from the tiles
+// before it (look-back) instead of a histogram pass over the indices and a
+// column scan.
+#ifndef CVLHIP_PART_LOOKBACK
+#define CVLHIP_PART_LOOKBACK 0
+#endif
+constexpr bool kPartLookback = CVLHIP_PART_LOOKBACK != 0;
 constexpr int kPartThreads = CVLHIP_PART_THREADS;
 constexpr int kPartItems = CVLHIP_PART_ITEMS;
 constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
@@ -58,6 +65,7 @@ constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
 constexpr int kPartV = 4;
 static_assert(kPartItems % kPartV == 0, "items per thread come in runs of kPartV");
 constexpr int kMaxFan = 256;                   // children per region per pass
+static_assert(!kPartLookback || kPartThreads >= kMaxFan, "the look-back gives every child a thread");
 static_assert(kMaxFan == kTileScanCols, "first-pass histogram rows are scanned 256 wide");
 static_assert(kBlkLog4 <= 16 && kBlkLog8 <= 16, "offsets within a final block fit the 16 bits the last pass keeps");
 
@@ -184,11 +192,29 @@ struct Local16 {
     }
 };
 
+// Look-back words of a first pass that chains its slot offsets from tile to
+// tile (one word per tile and child): the flag in bits 30-31, the count in
+// bits 0-29, so a word is written and read whole.
+constexpr uint32_t kLbAgg = 1u << 30, kLbPre = 2u << 30, kLbMask = (1u << 30) - 1;
+constexpr int kLbDepth = 8;
+__device__ __forceinline__ uint32_t ld_lb(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_lb(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One partition pass.  Region r = [r*psize, min((r+1)*psize, n)) of the
 // input is split into children of csize elements (csize divides psize,
 // psize/csize <= 256).  Block b handles tile (b % tiles_per_region) of
 // region (b / tiles_per_region).  Child c's pairs land in
 // [c*csize, c*csize + cnt[c]) of the output.
+// A first pass takes each tile's slot in each child from tile_ofs (the
+// column scan of per-tile histograms) or, with `lb`, from a look-back over
+// the tiles before it: tiles are then claimed in order from lb_ctr, every
+// tile publishes its counts and adds up the earlier ones child by child
+// (a tile waits only on tiles already claimed, so every wait ends), and the
+// last tile writes the children's totals to cnt.
 template <class T, bool FIRST, class Out = Pairs<T>>
 __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pairs<T> in,
                                                             const T *__restrict__ in_val, Out out,
@@ -197,10 +223,13 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
                                                             const unsigned *__restrict__ prev_cnt,
                                                             unsigned *__restrict__ cnt,
                                                             const unsigned *__restrict__ tile_ofs,
-                                                            unsigned *__restrict__ err) {
+                                                            unsigned *__restrict__ err,
+                                                            uint32_t *__restrict__ lb = nullptr,
+                                                            unsigned *__restrict__ lb_ctr = nullptr) {
     __shared__ unsigned s_hist[kMaxFan];
     __shared__ unsigned s_start[kMaxFan];
     __shared__ unsigned s_gofs[kMaxFan];
+    __shared__ unsigned s_bt;
     // The tile staged in bucket order: four-byte values share one 8-byte
     // word with their index (one LDS access each way, and the bucket is
     // recomputed from the index); wider values keep the index, the value
@@ -212,7 +241,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     __shared__ unsigned char s_dig[PACKED ? 1 : kPartTile];
 
     const int tid = threadIdx.x;
-    const int64_t bt = xcd_tile(blockIdx.x, gridDim.x);
+    for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
+    if (FIRST && lb && tid == 0) s_bt = atomicAdd(lb_ctr, 1u);
+    __syncthreads();
+    const int64_t bt = (FIRST && lb) ? (int64_t)s_bt : xcd_tile(blockIdx.x, gridDim.x);
     const int64_t region = bt / tiles_per_region;
     const int64_t chunk = bt % tiles_per_region;
     const int64_t rlo = region << plog;
@@ -220,9 +252,6 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     if (!FIRST && rlo + (int64_t)prev_cnt[region] < rhi) rhi = rlo + prev_cnt[region];   // pairs that landed
     const int64_t lo = rlo + chunk * kPartTile;
     const int64_t first_child = rlo >> clog;
-
-    for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
-    __syncthreads();
 
     int q[kPartItems];
     T v[kPartItems];
@@ -304,8 +333,54 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
             s_start[d] = run;
             run += c[i];
             if (tile_ofs) s_gofs[d] = tile_ofs[bt * kMaxFan + d];
-            else s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
+            else if (!(FIRST && lb)) s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
         }
+    }
+    if (FIRST && lb && tid < kMaxFan) {
+        // one thread per child: publish this tile's count, add up the
+        // earlier tiles' until one holds its inclusive prefix
+        const int d = tid;
+        const unsigned c = s_hist[d];
+        unsigned excl = 0;
+        if (bt == 0) {
+            st_lb(lb + d, kLbPre | c);
+        } else {
+            st_lb(lb + bt * kMaxFan + d, kLbAgg | c);
+            // kLbDepth words in flight per step: the published counts up to
+            // the first prefix are added; an unpublished word is read again
+            int64_t t = bt - 1;
+            int spins = 0;
+            for (;;) {
+                uint32_t w[kLbDepth];
+#pragma unroll
+                for (int j = 0; j < kLbDepth; j++) w[j] = t - j >= 0 ? ld_lb(lb + (t - j) * kMaxFan + d) : kLbPre;
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int j = 0; j < kLbDepth; j++) {
+                    if (done || used < j) continue;
+                    const uint32_t f = w[j] & ~kLbMask;
+                    if (f == 0) continue;
+                    excl += w[j] & kLbMask;
+                    used = j + 1;
+                    done = f == kLbPre;
+                }
+                if (done) break;
+                t -= used;
+                if (used) {
+                    spins = 0;
+                } else {
+                    if (++spins >= kSpinLimit) {
+                        atomicOr(err, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            st_lb(lb + bt * kMaxFan + d, kLbPre | ((excl + c) & kLbMask));
+        }
+        s_gofs[d] = excl;
+        if (bt == (int64_t)gridDim.x - 1) cnt[first_child + d] = excl + c;
     }
     __syncthreads();
 #pragma unroll
@@ -522,11 +597,14 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         }
     }
     const int64_t grid0 = (n + kPartTile - 1) / kPartTile;    // tiles of the first pass (one region)
+    // the column scan's status, or the look-back's tile counter; then the
+    // histogram, or the look-back words (cleared with the counters)
     uint64_t *status0 = (uint64_t *)b;
     b += up(8 * tile_scan_status_words(grid0));
-    fill_u32(cnts[0], 0, (int64_t)((char *)b - (char *)cnts[0]) / 4);
     unsigned *hist0 = (unsigned *)b;
-    b += up(4 * (int64_t)kMaxFan * grid0);
+    if (kPartLookback) b += up(4 * (int64_t)kMaxFan * grid0);
+    fill_u32(cnts[0], 0, (int64_t)((char *)b - (char *)cnts[0]) / 4);
+    if (!kPartLookback) b += up(4 * (int64_t)kMaxFan * grid0);
     int plog = nlog;
     int cur = -1;
     unsigned *cnt = nullptr, *prev = nullptr;
@@ -547,17 +625,23 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
         if (cur < 0) {
             // first pass: one region, many tiles -> slots from a histogram + column scan
             if (grid != grid0) usage_error("permute: first-pass tiling");
-            unsigned *hist = hist0;
-            uint64_t *status = status0;
-            k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
-            LAUNCHCHECK();
-            tile_scan(hist, grid, cnt, status, true);
+            unsigned *hist = kPartLookback ? nullptr : hist0;
+            uint32_t *lb = kPartLookback ? hist0 : nullptr;
+            unsigned *lb_ctr = kPartLookback ? (unsigned *)status0 : nullptr;
+            if (!kPartLookback) {
+                uint64_t *status = status0;
+                k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
+                LAUNCHCHECK();
+                tile_scan(hist, grid, cnt, status, true);
+            }
             if (last)
                 k_partition<T, true, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, Pairs<T>{}, (const T *)s, local[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+                    src, Pairs<T>{}, (const T *)s, local[out], n, plog, clog, tpr, nullptr, cnt, hist, err, lb,
+                    lb_ctr);
             else
                 k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err);
+                    src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err, lb,
+                    lb_ctr);
         } else {
             if (last)
                 k_partition<T, false, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(

@@ -25,7 +25,8 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libcvl_hip.so")
+# CVLHIP_LIBRARY names another build of the library (the autotuner's variants)
+LIB_PATH = os.environ.get("CVLHIP_LIBRARY") or os.path.join(HERE, "libcvl_hip.so")
 HEADERS = [os.path.join(ROOT, "include", "cvl.h"), os.path.join(ROOT, "include", "cvl_hip_ext.h")]
 
 _CTYPES = {

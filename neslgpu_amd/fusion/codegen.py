@@ -279,6 +279,18 @@ def generate(ops, library_name: str = "fused") -> str:
         a0_seg = bool(o.params) and o.params[0] == "SEGDES"
         gsrc = S.gather_sources(o.body)
         gseg = {v: k for k, v in gsrc.items()}
+        # the vectors of element length: the rules can only compare with
+        # the first five arguments
+        elem = [i for i, u in enumerate(o.params) if u != "SEGDES" and i < 5 and i not in gsrc]
+
+        def describes_elements(j):
+            # a descriptor over the elements: argument 1 itself defines the
+            # length, another one must describe a vector of that length (with
+            # none to compare with, the fuser's size analysis is the check)
+            if j == 0:
+                return [f"COMPAT{i + 1}" for i in elem]
+            return [f"COMPAT{elem[0] + 1}"] if elem else []
+
         for j, t in enumerate(o.params):
             bits = []
             if j in gsrc:
@@ -286,15 +298,10 @@ def generate(ops, library_name: str = "fused") -> str:
             elif j in gseg:
                 if gseg[j] < 5:
                     bits.append(f"COMPAT{gseg[j] + 1}")  # describes the gathered vector
-            elif j == 0:
-                if a0_seg:
-                    bits += [f"COMPAT{i + 1}" for i, u in enumerate(o.params)
-                             if u != "SEGDES" and i < 5 and i not in gsrc]
-            elif a0_seg:
-                if t == "SEGDES":
-                    bits.append("AGREE1")
-            else:
-                bits.append("COMPAT1" if t == "SEGDES" else "AGREE1")
+            elif t == "SEGDES":
+                bits += describes_elements(j)
+            elif j > 0 and not a0_seg:
+                bits.append("AGREE1")
             lens.append("|".join(bits) if bits else "NONE")
         rlen = "COMPAT1" if a0_seg else "AGREE1"
         scr = "1"
@@ -302,7 +309,7 @@ def generate(ops, library_name: str = "fused") -> str:
             r = o.reduce[1]
             if r >= 5:
                 raise S.DefinitionError("a reduced descriptor must be one of the first five arguments")
-            lens[r] = "COMPAT1"                        # describes the reduced elements
+            lens[r] = "|".join(describes_elements(r)) or "NONE"   # describes the reduced elements
             rlen = f"AGREE{r + 1}"                     # one result per segment
             scr = str(r + 1)                           # scratch sized by (n, m) of the descriptor
         out.append(f'    {{VCODE_FUSED_TOKEN, "fused{k}", {n}, 1, {{{atypes}}}, {{{", ".join(lens) or "NONE"}}}, '
@@ -327,7 +334,9 @@ def build(defs_text: str, out_so: str, hip_path: str | None = None, verbose: boo
     os.makedirs(os.path.dirname(os.path.abspath(out_so)), exist_ok=True)
     with open(hip_path, "w") as f:
         f.write(src)
+    tuned = os.path.join(PKG, "csrc", "tuned.hpp")      # the library's tile shapes (same tiles, same bits)
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+           *(["-include", tuned] if os.path.exists(tuned) else []),
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc"),
            hip_path, "-o", out_so + ".tmp",

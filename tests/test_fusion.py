@@ -7,10 +7,12 @@ import os
 
 import pytest
 
-from neslgpu_amd.build import FUSED_DIR, PROGRAMS_DIR, SUPEROPS_DIR
+from neslgpu_amd.build import FUSED_DIR, GEN_DIR, PROGRAMS_DIR, SUPEROPS_DIR
 from neslgpu_amd.fusion import codegen, superops as S
 
 DEFS = sorted(glob.glob(os.path.join(SUPEROPS_DIR, "*.superops")) + glob.glob(os.path.join(PROGRAMS_DIR, "*.superops")))
+# the definitions the fuser wrote at build time, each beside its library
+GEN_DEFS = sorted(glob.glob(os.path.join(GEN_DIR, "*.superops")))
 
 
 def test_parse_and_type():
@@ -65,10 +67,65 @@ class VopDes(C.Structure):
 TYPE = {"BOOL": 3, "INT": 4, "FLOAT": 5, "SEGDES": 6}
 
 
-@pytest.mark.parametrize("path", DEFS, ids=os.path.basename)
+def interpreter_accepts(o, arg_len, result_len):
+    """The length rules of a definition, applied as the interpreter applies
+    them (vcode/machine.cpp check_args, result_len) to arguments of
+    consistent sizes: n elements; every descriptor over the elements
+    describes n of them in a segment count of its own; a gathered source has
+    a length of its own, which its descriptor describes.  The rules must
+    accept them and give the result n elements (one per segment of the
+    reduced descriptor for a reduction)."""
+    n = 1000
+    gsrc = S.gather_sources(o.body)
+    gseg = {v: k for k, v in gsrc.items()}
+    args = []
+    for i, t in enumerate(o.params):
+        if i in gsrc:
+            args.append((5000 + i, None))                          # (len, seg_len)
+        elif t == "SEGDES":
+            args.append((7 + i, 5000 + gseg[i] if i in gseg else n))
+        else:
+            args.append((n, None))
+    for k, r in enumerate(arg_len):
+        for j in range(5):
+            if r & (1 << j) and j < len(args) and args[k][0] != args[j][0]:
+                return False
+            if r & (0x20 << j) and j < len(args) and args[k][1] != args[j][0]:
+                return False
+    want = args[o.reduce[1]][0] if o.reduce else n
+    for j in range(5):
+        if result_len & (1 << j):
+            return args[j][0] == want
+    for j in range(5):
+        if result_len & (0x20 << j):
+            return args[j][1] == want
+    return False
+
+
+def test_length_rules_of_descriptor_only_definitions():
+    """A reduction whose only argument is its descriptor (a +_REDUCE of a
+    DIST of a constant) and an elementwise tree over two descriptors of the
+    same elements in different segment counts."""
+    ops = S.parse("fused OP0#1 ($0 : SEGDES) = (+_REDUCE INT @ (DIST INT @ (CONST INT 5) $0) $0)\n"
+                  "fused OP1#1 ($0 : SEGDES, $1 : SEGDES) = "
+                  "(+ INT @ (DIST INT @ (CONST INT 1) $0) (DIST INT @ (CONST INT 2) $1))\n"
+                  "fused OP2#1 ($0 : INT, $1 : SEGDES, $2 : INT) = (+_REDUCE INT @ (* INT @ $0 $2) $1)\n")
+    src = codegen.generate(ops, "x")
+    rows = [l for l in src.splitlines() if l.strip().startswith('{VCODE_FUSED_TOKEN')]
+    lens = [l.split("}, {")[1] for l in rows]
+    assert lens == ["NONE", "NONE, NONE", "NONE, COMPAT1, AGREE1"]
+    rule = {"NONE": 0, "COMPAT1": 0x20, "AGREE1": 0x1}
+    for o, l, row in zip(ops, lens, rows):
+        res = row.split("}, {")[3]
+        res_rule = 1 << o.reduce[1] if o.reduce else 0x20
+        assert res == (f"AGREE{o.reduce[1] + 1}" if o.reduce else "COMPAT1")
+        assert interpreter_accepts(o, [rule[x] for x in l.split(", ")], res_rule)
+
+
+@pytest.mark.parametrize("path", DEFS + GEN_DEFS, ids=lambda p: os.path.relpath(p, FUSED_DIR))
 def test_built_library_tables(path):
     name = os.path.basename(path)[: -len(".superops")]
-    so = os.path.join(FUSED_DIR, name + ".so")
+    so = os.path.join(GEN_DIR if path in GEN_DEFS else FUSED_DIR, name + ".so")
     if not os.path.exists(so):
         pytest.skip("fused libraries not built")
     lib = C.CDLL(so)
@@ -83,8 +140,10 @@ def test_built_library_tables(path):
         assert d.cvl_desc == len(o.params)                 # ElwiseK
         if o.reduce:
             r = o.reduce[1]
-            assert d.result_len[0] == 1 << r and d.arg_len[r] == 0x20 and d.arg_for_scratch[0] == r + 1
+            assert d.result_len[0] == 1 << r and d.arg_for_scratch[0] == r + 1
         else:
             assert d.result_len[0] == (0x20 if o.params[0] == "SEGDES" else 0x1)
+        arg_len = [d.arg_len[i] for i in range(d.arg_num)]
+        assert interpreter_accepts(o, arg_len, d.result_len[0]), (o.number, arg_len)
         for k in ("", "_scratch", "_inplace"):
             getattr(lib, f"fused{o.number}{k}")

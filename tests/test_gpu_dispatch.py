@@ -198,3 +198,25 @@ def test_device_pick_is_the_networks_argmin(cv):
                 want = disp.choose(f)
                 got = L.cvlhip_segd_pick(sd, 1 if kind == "reduce" else 0, 8 if ty == "d" else 4)
                 assert got == want, (dist, param, log2n, kind, ty, got, want)
+
+
+def test_learned_dispatch_gain_on_held_out_workloads(cv):
+    """The built-in network on the held-out workloads (dispatch.EVAL_DISTS x
+    EVAL_SIZES: Zipf 1.4, 2.0, 2.8 and geometric(512) lengths, 2^17..2^25
+    elements; never used in training or in the choice of the training run):
+    the CVL entries' total time under the device's picks, plus what the pick
+    adds to mke_fov, is within 5% of the exhaustive autotuner's total (the
+    fastest arm timed for every case) and below the total of every fixed
+    arm."""
+    from neslgpu_amd import dispatch as D
+    L = cv.lib
+    if not L.cvlhip_sched_has_model():
+        pytest.skip("library built without a dispatcher model")
+    L.cvlhip_sched_enable(1)
+    ev = D.evaluate_live(cv, D.EVAL_DISTS, D.EVAL_SIZES, log=lambda s: None)
+    print("learned", ev["learned_total_ms"], "decision", ev["decision_total_ms"], "autotuned",
+          ev["autotuned_total_ms"], "fixed", ev["fixed_arm_total_ms"], flush=True)
+    assert ev["regret"] <= 0.05, ev["regret"]
+    for arm, total in ev["fixed_arm_total_ms"].items():
+        assert ev["learned_total_ms"] < total, (arm, total, ev["learned_total_ms"])
+    assert L.cvlhip_check_device_errors() == 0
