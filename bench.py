@@ -541,7 +541,20 @@ def main():
             compare_outputs(out["programs"], out["cpu_baseline"]["programs"])
     for rec in out.get("programs", {}).values():
         rec.pop("output", None)
+    out["children_at_exit"] = children()
     print(json.dumps(out), flush=True)
+
+
+def children() -> list:
+    """Command lines of this process's live descendants (bench.py starts
+    only vinterp runs and waits for each; any listed here is reported, not
+    left unexplained)."""
+    try:
+        import psutil
+        procs = psutil.Process().children(recursive=True)
+        return [" ".join(p.cmdline())[:200] for p in procs if p.is_running()]
+    except Exception as e:          # psutil missing or a process gone meanwhile
+        return [f"unknown: {e!r}"]
 
 
 if __name__ == "__main__":

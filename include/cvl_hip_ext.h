@@ -105,7 +105,8 @@ int cvlhip_sched_min_n(void);
 void cvlhip_sched_set_params(const float *params);
 void cvlhip_sched_get_params(float *params);
 int cvlhip_sched_has_model(void);
-/* 0: mke_fov picks nothing (every segmented op runs arm 0); 1: default. */
+/* 0: mke_fov picks nothing (every segmented op runs arm 0); 1: default.
+ * CVLHIP_SCHED=0 in the environment sets 0 when the library loads. */
 void cvlhip_sched_enable(int on);
 
 /* ---- learned dispatcher network (device pointers, fp32) -------------- */

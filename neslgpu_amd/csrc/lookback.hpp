@@ -89,8 +89,11 @@ template <> struct seq_combine<MulD> { static constexpr bool value = true; };
 // hold a tile with a full prefix before it can finish, and the prefix
 // frontier trails the newest tiles by more than 64 of them at full
 // occupancy, so it inspects two per lane.
+#ifndef CVLHIP_LOOKBACK_K
+#define CVLHIP_LOOKBACK_K 1
+#endif
 template <class Op>
-struct lookback_k { static constexpr int value = seq_combine<Op>::value ? 2 : 1; };
+struct lookback_k { static constexpr int value = seq_combine<Op>::value ? 2 : CVLHIP_LOOKBACK_K; };
 
 template <class Op>
 __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile, Op op,
@@ -178,6 +181,75 @@ __device__ typename Op::T lookback(const uint64_t *__restrict__ st, int64_t tile
         pos -= (int64_t)kWave * K;
     }
     return acc;
+}
+
+// ---------------------------------------------------------------- column chains
+// Per-tile counts of many columns (digits, buckets) chained from tile to
+// tile: one 32-bit word per (tile, column) holds a 2-bit state in bits
+// 30-31 and a count in bits 0-29 (counts below 2^30), so a word is written
+// and read whole.  Tile t publishes its own count (aggregate), adds up the
+// words of tiles t-1, t-2, ... until one holds an inclusive prefix, and
+// publishes its own inclusive prefix.  Tiles must be numbered in the order
+// they started (claimed from a counter), so every tile waited on is running
+// or done.
+//
+// Two state sets alternate between launches over the same words: every word
+// ends a launch in the `pre` state of its set, which is the `invalid` state
+// of the other set, so a buffer cleared to zero once serves any number of
+// launches with parity 0, 1, 0, ...
+struct ChainStates {
+    uint32_t invalid, agg, pre;
+};
+__device__ __forceinline__ ChainStates chain_states(int parity) {
+    return parity ? ChainStates{2u << 30, 3u << 30, 0u} : ChainStates{0u, 1u << 30, 2u << 30};
+}
+constexpr uint32_t kChainMask = (1u << 30) - 1;
+constexpr int kChainDepth = 8;          // words in flight per step of a walk
+
+// Called by the one thread of column d in tile t: publishes c and returns
+// the column's sum over tiles 0..t-1.  `cols` words per tile.
+__device__ __forceinline__ uint32_t chain_exclusive(uint32_t *words, int64_t t, int cols, int d, uint32_t c,
+                                                    int parity, unsigned *err) {
+    const ChainStates S = chain_states(parity);
+    auto ld = [](const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+    auto st = [](uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+    if (t == 0) {
+        st(words + d, S.pre | c);
+        return 0;
+    }
+    st(words + t * cols + d, S.agg | c);
+    uint32_t excl = 0;
+    int64_t u = t - 1;
+    int spins = 0;
+    for (;;) {
+        uint32_t w[kChainDepth];
+#pragma unroll
+        for (int j = 0; j < kChainDepth; j++) w[j] = u - j >= 0 ? ld(words + (u - j) * cols + d) : S.pre;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int j = 0; j < kChainDepth; j++) {
+            if (done || used < j) continue;          // stop at the first unpublished word
+            const uint32_t f = w[j] & ~kChainMask;
+            if (f == S.invalid) continue;
+            excl += w[j] & kChainMask;
+            used = j + 1;
+            done = f == S.pre;
+        }
+        if (done) break;
+        u -= used;
+        if (used) {
+            spins = 0;
+        } else {
+            if (++spins >= kSpinLimit) {
+                atomicOr(err, 2u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    st(words + t * cols + d, S.pre | ((excl + c) & kChainMask));
+    return excl;
 }
 
 }  // namespace cvlhip

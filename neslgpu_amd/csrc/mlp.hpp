@@ -37,12 +37,33 @@ __device__ __forceinline__ float sched_feature(int i, int64_t n, int64_t m, cons
     return 0.f;
 }
 
+// Schedules a descriptor admits whatever the network predicts.  The flat
+// schedules (0-2) take any descriptor.  The per-segment ones (3 thread, 4
+// wave, 5 workgroup) give a segment one lane, wave or workgroup: one long
+// segment serialises them and a few segments leave the chip idle, so each
+// is admitted only when the longest segment is below 2^(kArmMaxClass + 1)
+// and there are at least 2^kArmMinLogM segments.  cmax: the highest
+// length class holding a segment (-1: none).  Bit a set: schedule a admitted.
+constexpr int kArmMaxClass[3] = {5, 12, 15};
+constexpr int kArmMinLogM[3] = {14, 11, 9};
+__host__ __device__ inline unsigned sched_admitted(int cmax, int64_t m) {
+    unsigned a = 7u;
+    for (int k = 0; k < 3; k++)
+        if (cmax <= kArmMaxClass[k] && m >= ((int64_t)1 << kArmMinLogM[k])) a |= 1u << (3 + k);
+    return a;
+}
+__host__ __device__ inline int max_len_class(const uint32_t *segs) {
+    int c = kLenClasses - 1;
+    while (c >= 0 && segs[c] == 0) c--;
+    return c;
+}
+
 // Forward pass of one feature vector by one whole wave (lane i holds
-// feature i); every lane returns the index of the smallest prediction (the
-// first one on ties).  Values move between lanes with shuffles only, and
+// feature i); every lane returns the index of the smallest prediction among
+// the admitted schedules (the first one on ties).  Values move between lanes with shuffles only, and
 // sums run in index order, so the same parameters and features always give
 // the same pick.
-__device__ __forceinline__ int mlp_pick_wave(const float *__restrict__ P, float feature_of_lane) {
+__device__ __forceinline__ int mlp_pick_wave(const float *__restrict__ P, float feature_of_lane, unsigned admitted) {
     const int lane = lane_id();
     const int j1 = lane < kH ? lane : kH - 1;       // lanes past a layer's width compute a copy
     float a = P[kB1 + j1];
@@ -54,11 +75,11 @@ __device__ __forceinline__ int mlp_pick_wave(const float *__restrict__ P, float 
     const int j3 = lane < kA ? lane : kA - 1;
     a = P[kB3 + j3];
     for (int i = 0; i < kH; i++) a += P[kW3 + j3 * kH + i] * shfl(h2, i);
-    int best = 0;
+    int best = 0;                                   // flat8 is always admitted
     float by = shfl(a, 0);
     for (int k = 1; k < kA; k++) {
         const float v = shfl(a, k);
-        if (v < by) { by = v; best = k; }
+        if (((admitted >> k) & 1u) && v < by) { by = v; best = k; }
     }
     return best;
 }

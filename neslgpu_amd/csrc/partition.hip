@@ -26,6 +26,7 @@
 #include "segd.hpp"
 #include "partition.hpp"
 #include "colscan.hpp"
+#include "lookback.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
@@ -192,18 +193,6 @@ struct Local16 {
     }
 };
 
-// Look-back words of a first pass that chains its slot offsets from tile to
-// tile (one word per tile and child): the flag in bits 30-31, the count in
-// bits 0-29, so a word is written and read whole.
-constexpr uint32_t kLbAgg = 1u << 30, kLbPre = 2u << 30, kLbMask = (1u << 30) - 1;
-constexpr int kLbDepth = 8;
-__device__ __forceinline__ uint32_t ld_lb(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_lb(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // One partition pass.  Region r = [r*psize, min((r+1)*psize, n)) of the
 // input is split into children of csize elements (csize divides psize,
 // psize/csize <= 256).  Block b handles tile (b % tiles_per_region) of
@@ -337,48 +326,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
         }
     }
     if (FIRST && lb && tid < kMaxFan) {
-        // one thread per child: publish this tile's count, add up the
-        // earlier tiles' until one holds its inclusive prefix
+        // one thread per child: the counts of the tiles before this one
         const int d = tid;
         const unsigned c = s_hist[d];
-        unsigned excl = 0;
-        if (bt == 0) {
-            st_lb(lb + d, kLbPre | c);
-        } else {
-            st_lb(lb + bt * kMaxFan + d, kLbAgg | c);
-            // kLbDepth words in flight per step: the published counts up to
-            // the first prefix are added; an unpublished word is read again
-            int64_t t = bt - 1;
-            int spins = 0;
-            for (;;) {
-                uint32_t w[kLbDepth];
-#pragma unroll
-                for (int j = 0; j < kLbDepth; j++) w[j] = t - j >= 0 ? ld_lb(lb + (t - j) * kMaxFan + d) : kLbPre;
-                int used = 0;
-                bool done = false;
-#pragma unroll
-                for (int j = 0; j < kLbDepth; j++) {
-                    if (done || used < j) continue;
-                    const uint32_t f = w[j] & ~kLbMask;
-                    if (f == 0) continue;
-                    excl += w[j] & kLbMask;
-                    used = j + 1;
-                    done = f == kLbPre;
-                }
-                if (done) break;
-                t -= used;
-                if (used) {
-                    spins = 0;
-                } else {
-                    if (++spins >= kSpinLimit) {
-                        atomicOr(err, 2u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            st_lb(lb + bt * kMaxFan + d, kLbPre | ((excl + c) & kLbMask));
-        }
+        const unsigned excl = chain_exclusive(lb, bt, kMaxFan, d, c, 0, err);
         s_gofs[d] = excl;
         if (bt == (int64_t)gridDim.x - 1) cnt[first_child + d] = excl + c;
     }

@@ -11,8 +11,10 @@
 // the host launches only the passes that reorder something.
 //
 // Each reordering pass is a histogram kernel (per-tile digit counts), a
-// column scan of those counts (each tile's offset in each digit's bucket,
-// no atomics and no look-back chain) and a scatter kernel: a tile of 4096
+// column scan of those counts (each tile's offset in each digit's bucket)
+// and a scatter kernel -- or, with CVLHIP_RANK_CHAIN, the scatter kernel
+// alone, its tiles chaining the digit offsets with a look-back (tiles
+// claimed in order, one chained word per tile and digit): a tile of 4096
 // elements ranks its digits stably with a wave-level multisplit (8 ballots
 // give each lane the lanes holding the same digit), stages the tile in LDS in
 // digit order and writes each digit's run contiguously (keys, then
@@ -36,7 +38,15 @@ static_assert(kRadix == kTileScanCols, "per-tile digit counts are scanned 256 wi
 #ifndef CVLHIP_RANK_ITEMS
 #define CVLHIP_RANK_ITEMS 8
 #endif
+// 1: each pass chains its tiles' digit offsets with a look-back inside the
+// scatter kernel (tiles claimed in order) instead of a per-tile histogram
+// kernel and a column scan before it.
+#ifndef CVLHIP_RANK_CHAIN
+#define CVLHIP_RANK_CHAIN 0
+#endif
+constexpr bool kRankChain = CVLHIP_RANK_CHAIN != 0;
 constexpr int kRankThreads = CVLHIP_RANK_THREADS;        // waves of 64
+static_assert(kRankThreads >= 256, "one thread per digit");
 constexpr int kRankItems = CVLHIP_RANK_ITEMS;            // per lane
 constexpr int kRankTile = kRankThreads * kRankItems;     // keys per tile
 constexpr int kWaveSpan = kWave * kRankItems;            // consecutive keys per wave
@@ -133,6 +143,9 @@ struct PassArgs {
     int *idx_out;            // the element order after this pass
     const unsigned *starts;  // 256 bucket starts of this pass
     const unsigned *tile_ofs;  // tiles x 256: elements of each digit in earlier tiles
+    uint32_t *chain;           // or null: tiles x 256 chain words (chain_exclusive)
+    unsigned *ticket;          // the chain's tile counter
+    int parity;                // the chain's state set
     int64_t n;
     int shift;               // digit = (field >> shift) & 255
     // 0: int values, 1: double values, 2: keys32, 3: keys64, 4: segment
@@ -237,12 +250,16 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     __shared__ unsigned s_gbase[kRadix];      // global position of the run
     __shared__ KeyT s_stage[kRankTile];       // keys, then (reinterpreted) indices, in digit order
     __shared__ unsigned char s_dig[kRankTile];
+    __shared__ unsigned s_tile;
 
     const int tid = threadIdx.x;
     const int w = tid / kWave, lane = tid % kWave;
     for (int i = tid; i < W * kRadix; i += kRankThreads) (&s_cnt[0][0])[i] = 0;
+    // a chained pass numbers its tiles in the order they start; the
+    // order of the tiles' runs within a digit is then their data order
+    if (a.chain && tid == 0) s_tile = atomicAdd(a.ticket, 1u);
     __syncthreads();
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t tile = a.chain ? (int64_t)s_tile : xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = tile * kRankTile;
     const int64_t count = a.n - t0 < kRankTile ? a.n - t0 : kRankTile;
     const int64_t wb = t0 + (int64_t)w * kWaveSpan;
@@ -340,7 +357,8 @@ loaded:
             run += c;
         }
         s_tot[d] = run;
-        const unsigned prefix = a.tile_ofs[tile * kRadix + d];
+        const unsigned prefix = a.chain ? chain_exclusive(a.chain, tile, kRadix, d, run, a.parity, a.err)
+                                        : a.tile_ofs[tile * kRadix + d];
         s_gbase[d] = a.starts[d] + prefix;
     }
     __syncthreads();
@@ -500,7 +518,13 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     for (int i = 0; i < nrun; i++) if (run[i] < key_passes) last_key_run = i;
 
     const int64_t sw = tile_scan_status_words(tiles);
-    fill_u32(scan_status, 0, 2 * sw * nrun);       // the status regions of every pass that runs
+    // Chained passes: per-digit counts below 2^30 (n < 2^30); one set of
+    // words for all passes (alternating state sets) and a tile counter per
+    // pass, contiguous and cleared together.
+    const bool chain = kRankChain && n < (int64_t)kChainMask;
+    unsigned *tickets = (unsigned *)(b + L.counters);
+    if (chain) fill_u32(tickets, 0, (L.status - L.counters) / 4 + kRadix * tiles);
+    else fill_u32(scan_status, 0, 2 * sw * nrun);       // the status regions of every pass that runs
     void *keys[2] = {b + L.keysA, b + L.keysB};
     int *idxs[2] = {(int *)(b + L.idxA), (int *)(b + L.idxB)};
     int cur = -1;   // buffer holding the current (keys, idx); -1 = original input
@@ -515,6 +539,9 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         a.seg = seg;
         a.starts = starts + p * kRadix;
         a.tile_ofs = tile_ofs;
+        a.chain = chain ? (uint32_t *)tile_ofs : nullptr;
+        a.ticket = tickets + i;
+        a.parity = i & 1;
         a.idx_in = cur < 0 ? nullptr : idxs[cur];
         if (p < key_passes) {
             a.source = cur < 0 ? (dbl ? 1 : 0) : (dbl ? 3 : (kbytes == 4 ? 2 : (kbytes == 2 ? 5 : 6)));
@@ -538,10 +565,12 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         }
         a.idx_out = idxs[out];
         a.err = device_error_word();
-        if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
-        else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
-        LAUNCHCHECK();
-        tile_scan(tile_ofs, tiles, nullptr, scan_status + i * sw, true);
+        if (!chain) {
+            if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
+            else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
+            LAUNCHCHECK();
+            tile_scan(tile_ofs, tiles, nullptr, scan_status + i * sw, true);
+        }
         const dim3 grid((unsigned)tiles), block(kRankThreads);
         if (dbl) {
             if (a.keys_out) k_rank_pass<uint64_t, 8><<<grid, block, 0, g_stream>>>(a);

@@ -142,10 +142,16 @@ static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // double add scans of 2^27 and of 2^26 in Zipf(1.5) segments run best at
 // 1024 (0.537 and 0.317 ms against 0.678 and 0.332 at 512).  Scratch
 // sizes assume 256 threads and 16 rows, the most tiles.
+#ifndef CVLHIP_SCAN_BLOCK_U4
+#define CVLHIP_SCAN_BLOCK_U4 1024
+#endif
+#ifndef CVLHIP_SCAN_ROWS_U4
+#define CVLHIP_SCAN_ROWS_U4 12
+#endif
 template <class Op, bool SEG, bool REDUCE>
 constexpr int scan_block() {
     if constexpr (REDUCE) return kBlock;
-    else if constexpr (sizeof(typename Op::T) == 4) return SEG ? 512 : 1024;
+    else if constexpr (sizeof(typename Op::T) == 4) return SEG ? 512 : CVLHIP_SCAN_BLOCK_U4;
     else return 1024;
 }
 
@@ -157,7 +163,9 @@ constexpr int scan_block() {
 // 0.443 ms but slow the others (segmented int 0.485 -> 0.543, double
 // 0.536 -> 0.652), so only plain 4-byte scans use them.
 template <class Op, bool SEG, bool REDUCE>
-constexpr int scan_rows() { return REDUCE ? kScanRows : ((sizeof(typename Op::T) == 4 && !SEG) ? 12 : 8); }
+constexpr int scan_rows() {
+    return REDUCE ? kScanRows : ((sizeof(typename Op::T) == 4 && !SEG) ? CVLHIP_SCAN_ROWS_U4 : 8);
+}
 
 // err: the device error word the kernels raise (null: the library's).
 template <class Op, int R, bool SEG, bool REDUCE, int B = scan_block<Op, SEG, REDUCE>()>
@@ -368,11 +376,12 @@ using AddMaxD = PairOp<AddD, MaxD>;
 #endif
 constexpr int kPairBlock = CVLHIP_PAIR_BLOCK;
 
-// Counts the tiles of the shape the pass is launched with.
+// Counts the tiles of the shape the pass is launched with: the scan's
+// status words, then the reduction's records.
 int64_t add_max_pair_scratch_bytes(int64_t n) {
     using S = ScanShape<AddMaxD, kPairRows, kPairBlock>;
     const int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    return kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8 + 256;
+    return kScanHeaderBytes + split_status_bytes<AddMaxD>(tiles) + NolbRecords<double>::bytes(tiles) + 256;
 }
 
 void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd, int64_t n, int64_t m,
@@ -388,13 +397,18 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     if (n <= 0) return;
     const int64_t tiles = (n + S::TILE - 1) / S::TILE;
     if (tiles > 0x7fffffff) usage_error("vector too long");
-    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + tiles * TileStatus<PairD>::kWords * 8) / 4);
+    // the scan's status words are cleared (the records are all written)
+    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + split_status_bytes<AddMaxD>(tiles)) / 4);
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
-    k_scan<AddMaxD, kPairRows, true, true, PairIn, kPairBlock><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
+    k_scan<AddMaxD, kPairRows, true, true, PairIn, kPairBlock, true><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
         PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter, al16(s),
         device_error_word());
     LAUNCHCHECK();
+    if (tiles > 1) {
+        k_seg_reduce_fix<MaxD><<<1, kFixThreads, 0, g_stream>>>(split_records<AddMaxD>(status, tiles), tiles, red_d);
+        LAUNCHCHECK();
+    }
 }
 
 #define CVLHIP_INST_SEG(OP)                                                                         \

@@ -119,6 +119,8 @@ struct PairD {
 template <class A, class B>
 struct PairOp {
     using T = PairD;
+    using OpA = A;
+    using OpB = B;
     __host__ __device__ static T neutral() { return {A::neutral(), B::neutral()}; }
     __host__ __device__ static T cvl_id() { return {A::cvl_id(), B::cvl_id()}; }
     __host__ __device__ T operator()(T x, T y) const { return {A()(x.a, y.a), B()(x.b, y.b)}; }
@@ -201,23 +203,38 @@ struct NolbRecords {
 // dual loader (SEG=true, REDUCE=true), both at once on a pair.
 // NOLB (segmented reductions only): no look-back; `status` holds the
 // NolbRecords of gridDim.x tiles and k_seg_reduce_fix finishes the
-// segments that cross tiles.
+// segments that cross tiles.  A dual pass with NOLB splits the pair: the
+// scan (component a) chains its tiles with a look-back on its own, 8-byte
+// values (status words first, split_records() after them), and the
+// reduction (component b) runs without look-back, finished by
+// k_seg_reduce_fix<OpB> on its records.
 //
 // scan_tile is the work of one workgroup on one tile (`tile` of `ntiles`;
 // NOLB records are laid out for ntiles tiles).  k_scan runs one tile per
 // workgroup; the segmented-op schedules (k_seg_sched) loop over tiles.
+// Where a split dual pass keeps the reduction's records: after the status
+// words of its ntiles tiles.
+template <class Op>
+__host__ __device__ inline int64_t split_status_bytes(int64_t ntiles) {
+    return (ntiles * TileStatus<double>::kWords * 8 + 255) & ~(int64_t)255;
+}
+template <class Op>
+__host__ __device__ inline NolbRecords<double> split_records(void *status, int64_t ntiles) {
+    return NolbRecords<double>::at((char *)status + split_status_bytes<Op>(ntiles), ntiles);
+}
+
 template <class Op, int R, bool SEG, bool REDUCE, class In, int B, bool NOLB>
 __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                           uint64_t *__restrict__ status, int64_t tile, int64_t ntiles,
                                           bool aligned, unsigned *__restrict__ err) {
     using T = typename Op::T;
     using S = ScanShape<Op, R, B>;
-    using St = TileStatus<T>;
     constexpr int V = S::V;
     constexpr uint32_t VMASK = (1u << V) - 1u;
     constexpr bool DUAL = io_dual<In>::value;
+    constexpr bool SPLIT = DUAL && NOLB;     // look-back on component a only
     static_assert(!DUAL || (SEG && REDUCE), "a dual pass is a segmented scan and reduction");
-    static_assert(!NOLB || (SEG && REDUCE && !DUAL), "only segmented reductions run without look-back");
+    static_assert(!NOLB || (SEG && REDUCE), "only segmented reductions run without look-back");
     const Op op;
 
     __shared__ T s_wagg[S::WAVES];
@@ -354,15 +371,35 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             tf |= f;
         }
         T P = Op::neutral();
-        if constexpr (NOLB) {
+        if constexpr (SPLIT) {
+            using A = typename Op::OpA;
+            using Sa = TileStatus<double>;
+            if (lane == 0) {
+                const NolbRecords<double> rec = split_records<Op>(status, ntiles);
+                rec.agg[tile] = tv.b;
+                rec.flag[tile] = tf;
+            }
+            if (tile == 0) {
+                if (lane == 0) Sa::publish(status, 0, kPrefix, tv.a);
+            } else {
+                const bool done = tf;
+                if (lane == 0) Sa::publish(status, tile, done ? kPrefix : kAggregate, tv.a);
+                const int64_t t0 = tile * S::TILE;
+                const bool first_head = (sb.heads[t0 >> 5] >> (t0 & 31)) & 1u;
+                if (!first_head) P.a = lookback(status, tile, A(), err);
+                if (!done && lane == 0) Sa::publish(status, tile, kPrefix, A()(P.a, tv.a));
+            }
+        } else if constexpr (NOLB) {
             if (lane == 0) {
                 const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
                 rec.agg[tile] = tv;
                 rec.flag[tile] = tf;
             }
         } else if (tile == 0) {
+            using St = TileStatus<T>;
             if (lane == 0) St::publish(status, 0, kPrefix, tv);
         } else {
+            using St = TileStatus<T>;
             const bool done = SEG && tf;      // a head inside the tile: later tiles need nothing earlier
             if (lane == 0) St::publish(status, tile, done ? kPrefix : kAggregate, tv);
             bool first_head = false;
@@ -405,7 +442,15 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
                     const uint32_t wd = sb.heads[e >> 5];
                     const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);
                     const int rk = sb.wcnt[e >> 5] + __popc(wd & mask);
-                    in.red[sb.nz[rk - 1]] = y[k].b;
+                    const int slot = sb.nz[rk - 1];
+                    if (SPLIT && ((needs >> (r * V + k)) & 1ull) && !s_winf[w]) {
+                        // the reduction of the segment that entered the tile
+                        // lacks the tile's incoming prefix: recorded
+                        s_fix_slot = slot;
+                        if constexpr (NOLB) s_fix_val = pv[k];
+                    } else {
+                        in.red[slot] = y[k].b;
+                    }
                 }
             }
         } else if constexpr (!REDUCE) {
@@ -431,9 +476,15 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
     if constexpr (NOLB) {
         __syncthreads();
         if (threadIdx.x == 0) {
-            const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
-            rec.slot[tile] = s_fix_slot;
-            rec.val[tile] = s_fix_val;
+            if constexpr (SPLIT) {
+                const NolbRecords<double> rec = split_records<Op>(status, ntiles);
+                rec.slot[tile] = s_fix_slot;
+                rec.val[tile] = s_fix_val.b;
+            } else {
+                const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
+                rec.slot[tile] = s_fix_slot;
+                rec.val[tile] = s_fix_val;
+            }
         }
     }
 }

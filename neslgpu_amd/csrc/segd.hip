@@ -16,6 +16,7 @@
 #include "../../include/cvl_hip_ext.h"
 
 #include <stddef.h>
+#include <stdlib.h>
 
 // The trained dispatcher network, generated at build time from
 // neslgpu_amd/dispatch_model.npz (sched_model.inc in the build directory).
@@ -35,7 +36,13 @@ static int g_sched_model = 1;
 __device__ float g_sched_params[kParams];
 static int g_sched_model = 0;
 #endif
-static int g_sched_enabled = 1;
+// CVLHIP_SCHED=0 in the environment starts the library with the
+// dispatcher off (every segmented op then runs the flat 8-row schedule).
+static int sched_enabled_at_load() {
+    const char *e = getenv("CVLHIP_SCHED");
+    return !(e && e[0] == '0' && e[1] == 0);
+}
+static int g_sched_enabled = sched_enabled_at_load();
 
 constexpr int kHist = 2 * kLenClasses;
 
@@ -56,7 +63,7 @@ __device__ void sched_pick_block(SegdHeader *hdr, int64_t n, int64_t m, const un
     if (w < 4) {
         const bool red = (w >> 1) != 0, dbl = (w & 1) != 0;
         const float f = sched_feature(lane < kF ? lane : 0, n, m, hist, hist + kLenClasses, dbl, red);
-        const int a = mlp_pick_wave(g_sched_params, f);
+        const int a = mlp_pick_wave(g_sched_params, f, sched_admitted(max_len_class(hist + kLenClasses), m));
         if (lane == 0) s_pick[w] = (unsigned)a;
     }
     __syncthreads();

@@ -40,9 +40,12 @@ enum : int {
 
 // Threads per workgroup: those of the fixed schedule of each kind
 // (segmented 4-byte scans 512, 8-byte scans 1024, reductions 512).
+#ifndef CVLHIP_SEG_SCAN_BLOCK4
+#define CVLHIP_SEG_SCAN_BLOCK4 512
+#endif
 template <class Op, bool REDUCE>
 __host__ __device__ constexpr int sched_block() {
-    return REDUCE ? kSegReduceBlock : (sizeof(typename Op::T) == 4 ? 512 : 1024);
+    return REDUCE ? kSegReduceBlock : (sizeof(typename Op::T) == 4 ? CVLHIP_SEG_SCAN_BLOCK4 : 1024);
 }
 
 template <class Op, bool REDUCE, int R>

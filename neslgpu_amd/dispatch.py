@@ -55,7 +55,13 @@ def mlp_dims(lib):
 
 # ------------------------------------------------------------------ workloads
 def lengths(kind: str, n: int, param: float, rng: np.random.Generator, lmax: int = 1 << 20) -> np.ndarray:
-    """Segment lengths summing exactly to n."""
+    """Segment lengths summing exactly to n ("few": n split into `param`
+    equal segments, not clipped to lmax)."""
+    if kind == "few":
+        k = int(param)
+        L = np.full(k, n // k, np.int64)
+        L[: n % k] += 1
+        return L[L > 0].astype(np.int32)
     out, tot = [], 0
     while tot < n:
         k = max(1024, int(2 * (n - tot) / max(param, 1.0)) + 16)
@@ -86,7 +92,8 @@ TRAIN_DISTS = ([("geometric", m) for m in (2, 8, 64, 1024, 8192)]
                + [("uniform", m) for m in (3, 200, 3000)]
                + [("zipf", a) for a in (1.3, 1.7, 2.2, 3.0)]
                + [("const", c) for c in (1, 32, 100000)]
-               + [("bimodal", 0)])
+               + [("bimodal", 0)]
+               + [("few", k) for k in (2, 16, 300)])
 TRAIN_SIZES = [1 << k for k in (15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25)]
 VALID_DISTS = [("zipf", 1.6), ("geometric", 256), ("uniform", 40)]
 VALID_SIZES = [1 << k for k in (16, 18, 20, 22, 24)]
@@ -148,6 +155,33 @@ def features(cv, sd, kind: str, ty: str) -> np.ndarray:
     return out
 
 
+# Schedules a descriptor admits (csrc/mlp.hpp sched_admitted): the thread,
+# wave and workgroup arms only below a longest-segment class and above a
+# segment count; the flat arms always.
+ARM_MAX_CLASS = {3: 5, 4: 12, 5: 15}
+ARM_MIN_LOG_M = {3: 14, 4: 11, 5: 9}
+
+
+def admitted(feat: np.ndarray) -> np.ndarray:
+    """Boolean mask over ARMS for feature vectors (rows of `feat`), from
+    the segment-class fractions and log2(m)/32 the library computes."""
+    feat = np.atleast_2d(feat)
+    classes = (feat.shape[1] - 5) // 2
+    segs = feat[:, 3 + classes: 3 + 2 * classes]
+    present = segs > 0
+    cmax = np.where(present.any(axis=1), classes - 1 - np.argmax(present[:, ::-1], axis=1), -1)
+    logm = feat[:, 1].astype(np.float64) * 32.0
+    ok = np.ones((feat.shape[0], len(ARMS)), bool)
+    for a in ARM_MAX_CLASS:
+        ok[:, a] = (cmax <= ARM_MAX_CLASS[a]) & (logm >= ARM_MIN_LOG_M[a] - 1e-6)
+    return ok
+
+
+def masked_argmin(pred: np.ndarray, ok: np.ndarray) -> np.ndarray:
+    """First index of the smallest admitted prediction per row."""
+    return np.where(ok, pred, np.inf).argmin(axis=1)
+
+
 # ------------------------------------------------------------------ the network
 class Dispatcher:
     """MLP over descriptor statistics -> predicted log-time per arm."""
@@ -205,7 +239,7 @@ class Dispatcher:
         return losses
 
     def choose(self, feat: np.ndarray) -> int:
-        return int(np.argmin(self.predict(feat[None, :])[0]))
+        return int(masked_argmin(self.predict(feat[None, :]), admitted(feat))[0])
 
     def install(self):
         """Make the library's mke_fov pick with these parameters."""
@@ -267,7 +301,7 @@ def gather(cv, dists, sizes, kinds=KINDS, seed: int = 0, log=print, reps: int = 
 def offline_regret(disp: Dispatcher, X, T):
     """Regret of the network's picks on measured arm times (no decision cost)."""
     times = np.exp(T)
-    pick = disp.predict(X).argmin(axis=1)
+    pick = masked_argmin(disp.predict(X), admitted(X))
     chosen = times[np.arange(len(X)), pick]
     return float(chosen.sum() / times.min(axis=1).sum() - 1.0)
 

@@ -41,18 +41,42 @@ RECORD = os.path.join(ROOT, "profiles", "tile_autotune.md")
 
 # macro -> (candidate values, the primitive whose time judges it)
 SPACE = {
+    "CVLHIP_SCAN_ROWS_U4,CVLHIP_SCAN_BLOCK_U4": ([(12, 1024), (8, 1024), (8, 512), (12, 512), (16, 512), (16, 256)],
+                                                 "add_suz"),
+    "CVLHIP_LOOKBACK_K": ([1, 2, 4], "add_suz"),
+    "CVLHIP_SEG_SCAN_BLOCK4": ([256, 512, 1024], "add_sez"),
     "CVLHIP_REDUCE_GRID": ([1024, 2048], "max_ruz"),
     "CVLHIP_PACK_ROWS": ([4, 6, 8, 10], "pk2_luz"),
     "CVLHIP_PART_ITEMS": ([4, 8, 12], "smp_puz"),
     "CVLHIP_PART_THREADS": ([512, 1024], "smp_puz"),
-    "CVLHIP_PART_LOOKBACK": ([0, 1], "smp_puz"),
     "CVLHIP_BLK_LOG4": ([12, 13, 14], "smp_puz"),
     "CVLHIP_RANK_THREADS": ([256, 512, 1024], "rku_luz"),
     "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
+    "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
     "CVLHIP_SEG_REDUCE_BLOCK": ([256, 512, 1024], "max_red"),
-    "CVLHIP_PAIR_ROWS": ([4, 8], "add_sed_max_red"),
-    "CVLHIP_PAIR_BLOCK": ([256, 512], "add_sed_max_red"),
+    # a comma-joined key searches its macros together (values are tuples)
+    "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(r, b) for r in (4, 8) for b in (256, 512, 1024)], "add_sed_max_red"),
 }
+
+
+def macros(key: str) -> list:
+    return key.split(",")
+
+
+def key_value(key: str):
+    """Current value of a SPACE key (a tuple for a joined key)."""
+    vals = tuple(current_value(m)[0] for m in macros(key))
+    return vals if len(vals) > 1 else vals[0]
+
+
+def norm(v):
+    """Values compare as tuples (JSON gives joined values back as lists)."""
+    return tuple(v) if isinstance(v, (list, tuple)) else v
+
+
+def defines(key: str, value) -> list:
+    vals = value if isinstance(value, (tuple, list)) else (value,)
+    return [f"-D{m}={v}" for m, v in zip(macros(key), vals)]
 
 
 # ------------------------------------------------------------------ build host
@@ -91,8 +115,9 @@ def includers(path: str) -> list:
     return sorted(out)
 
 
-def variant_name(macro: str, value) -> str:
-    return f"{macro[len('CVLHIP_'):].lower()}_{value}"
+def variant_name(key: str, value) -> str:
+    vals = value if isinstance(value, (tuple, list)) else (value,)
+    return "_".join(f"{m[len('CVLHIP_'):].lower()}_{v}" for m, v in zip(macros(key), vals))
 
 
 def build_variants(space=SPACE, verbose=True) -> dict:
@@ -102,10 +127,14 @@ def build_variants(space=SPACE, verbose=True) -> dict:
     B.build(verbose=False)                     # the default objects the variants link with
     jobs = []
     for macro, (values, prim) in space.items():
-        cur, src = current_value(macro)
-        files = includers(src) if src else sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+        cur = key_value(macro)
+        files = set()
+        for m in macros(macro):
+            src = current_value(m)[1]
+            files |= set(includers(src) if src else glob.glob(os.path.join(CSRC, "*.hip")))
+        files = sorted(files)
         for v in values:
-            if v == cur:
+            if norm(v) == norm(cur):
                 continue
             jobs.append((macro, v, prim, files))
 
@@ -119,7 +148,7 @@ def build_variants(space=SPACE, verbose=True) -> dict:
         objs = {os.path.basename(o): o for o in glob.glob(os.path.join(B.OBJ, "*.hip.o"))}
         for f in files:
             o = os.path.join(od, os.path.basename(f) + ".o")
-            r = subprocess.run([B.HIPCC, *B.FLAGS, *B.extra_flags(), f"-D{macro}={v}", "-I", B.OBJ, "-c", f,
+            r = subprocess.run([B.HIPCC, *B.FLAGS, *B.extra_flags(), *defines(macro, v), "-I", B.OBJ, "-c", f,
                                 "-o", o], capture_output=True, text=True)
             if r.returncode != 0:
                 return name, {"macro": macro, "value": v, "primitive": prim, "error": r.stderr[-800:]}
@@ -134,7 +163,7 @@ def build_variants(space=SPACE, verbose=True) -> dict:
         res = dict(ex.map(one, jobs))
     res["default"] = {"macro": None, "value": None, "primitive": None,
                       "lib": os.path.relpath(B.LIB, ROOT),
-                      "defaults": {m: current_value(m)[0] for m in space}}
+                      "defaults": {m: key_value(m) for m in space}}
     with open(os.path.join(TUNE_DIR, "variants.json"), "w") as f:
         json.dump(res, f, indent=1)
     if verbose:
@@ -180,10 +209,14 @@ def measure(primitive: str, reps: int = 7) -> dict:
     ok = True
     # correctness, small (several tiles of every shape searched)
     n = 3_000_017
-    if primitive in ("max_ruz", "pk2_luz", "smp_puz", "rku_luz"):
+    if primitive in ("add_suz", "max_ruz", "pk2_luz", "smp_puz", "rku_luz"):
         x = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
         dx = torch.from_numpy(x).to(dev)
-        if primitive == "max_ruz":
+        if primitive == "add_suz":
+            d = torch.empty_like(dx)
+            L.add_suz(d, dx, n, cv.scratch(L.add_suz_scratch(n)))
+            ok = np.array_equal(d.cpu().numpy(), O.scan("add", "z", x))
+        elif primitive == "max_ruz":
             ok = L.max_ruz(dx, n, cv.scratch(L.max_ruz_scratch(n))) == int(x.max())
         elif primitive == "pk2_luz":
             f = (rng.random(n) < 0.5).astype(np.int32)
@@ -205,9 +238,30 @@ def measure(primitive: str, reps: int = 7) -> dict:
                 L.smp_pud(e, dy, dp, k, cv.scratch(L.smp_pud_scratch(k)))
                 ok = ok and np.array_equal(e.cpu().numpy(), O.smp_pu("d", y, p))
         else:
+            # int up, double down with ties, segmented int
             d = torch.empty_like(dx)
             L.rku_luz(d, dx, n, cv.scratch(L.rku_luz_scratch(n)))
             ok = np.array_equal(d.cpu().numpy(), O.rank(x, True))
+            y = np.round(rng.uniform(-1e3, 1e3, n), 1)
+            L.rkd_lud(d, torch.from_numpy(y).to(dev), n, cv.scratch(L.rkd_lud_scratch(n)))
+            ok = ok and np.array_equal(d.cpu().numpy(), O.rank(y, False))
+            lens, sd = seg_case(n)
+            L.rku_lez(d, dx, sd, n, lens.size, cv.scratch(L.rku_lez_scratch(n, lens.size)))
+            ok = ok and np.array_equal(d.cpu().numpy(), O.rank(x, True, lens))
+    elif primitive == "add_sez":
+        # the dispatched schedule and each flat arm, on two segment shapes
+        for lens in (D.lengths("geometric", n, 256, rng), D.lengths("zipf", n, 1.5, rng, lmax=1 << 20)):
+            sd, m = cv.segd(lens), lens.size
+            x = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
+            dx = torch.from_numpy(x).to(dev)
+            want = O.segscan("add", "z", x, lens)
+            d = torch.empty_like(dx)
+            L.add_sez(d, dx, sd, n, m, cv.scratch(L.add_sez_scratch(n, m)))
+            ok = ok and np.array_equal(d.cpu().numpy(), want)
+            scr = cv.scratch(L.cvlhip_seg_scratch(n, m))
+            for arm in range(3):
+                L.cvlhip_seg_scan(D.OPS["add"], 0, arm, d, dx, sd, n, m, scr)
+                ok = ok and np.array_equal(d.cpu().numpy(), want)
     else:
         lens, sd = seg_case(n)
         m = lens.size
@@ -241,7 +295,15 @@ def measure(primitive: str, reps: int = 7) -> dict:
         s = torch.empty(n, dtype=torch.int32, device=dev)
         L.cvlhip_fill_splitmix_u32(s, n, 0x4E45534C00000002)
         d = torch.empty_like(s)
-        if primitive == "max_ruz":
+        if primitive == "add_suz":
+            scr = cv.scratch(L.add_suz_scratch(n))
+            fn = lambda: L.add_suz(d, s, n, scr)
+        elif primitive == "add_sez":
+            lens = D.lengths("geometric", n, 256, np.random.default_rng(4))
+            sd, m = cv.segd(lens), lens.size
+            scr = cv.scratch(L.add_sez_scratch(n, m))
+            fn = lambda: L.add_sez(d, s, sd, n, m, scr)
+        elif primitive == "max_ruz":
             scr = cv.scratch(L.max_ruz_scratch(n))
             fn = lambda: L.max_ruz(s, n, scr)
         elif primitive == "pk2_luz":
@@ -300,10 +362,10 @@ def pick(path: str) -> dict:
     chosen, rows = {}, []
     for macro, (values, prim) in SPACE.items():
         base = res.get(f"default:{prim}", {})
-        best_v, best_ms = defaults[macro], base.get("ms") if base.get("correct") else None
-        rows.append((macro, defaults[macro], prim, base.get("ms"), base.get("correct"), "current"))
+        best_v, best_ms = norm(defaults[macro]), base.get("ms") if base.get("correct") else None
+        rows.append((macro, best_v, prim, base.get("ms"), base.get("correct"), "current"))
         for v in values:
-            if v == defaults[macro]:
+            if norm(v) == norm(defaults[macro]):
                 continue
             r = res.get(variant_name(macro, v))
             if not r:
@@ -319,8 +381,10 @@ def pick(path: str) -> dict:
              "// source by neslgpu_amd/build.py.  Each value is the fastest correct",
              "// one of its macro's candidates on MI355X, the others at their values.",
              "#pragma once"]
-    for m, v in chosen.items():                 # a -D on the command line still wins
-        lines += [f"#ifndef {m}", f"#define {m} {v}", "#endif"]
+    for key, value in chosen.items():           # a -D on the command line still wins
+        vals = value if isinstance(value, tuple) else (value,)
+        for m, v in zip(macros(key), vals):
+            lines += [f"#ifndef {m}", f"#define {m} {v}", "#endif"]
     with open(TUNED, "w") as f:
         f.write("\n".join(lines) + "\n")
     md = ["# Tile-shape autotuning (MI355X)", "",

@@ -130,7 +130,8 @@ def forcing_params(dims, arm):
 
 def test_forced_picks_run_through_cvl_entries(cv):
     """Each arm, picked by mke_fov, is what add_sez / add_sed / max_rez /
-    min_red / mul_rez run; results match the oracle."""
+    min_red / mul_rez run; results match the oracle.  The lengths (1..40,
+    with empty segments, 20k segments) admit every arm."""
     import torch
     from neslgpu_amd import dispatch as D
     L = cv.lib
@@ -139,10 +140,10 @@ def test_forced_picks_run_through_cvl_entries(cv):
     L.cvlhip_sched_get_params(saved)
     had = L.cvlhip_sched_has_model()
     rng = np.random.default_rng(21)
-    n = 3 * L.cvlhip_sched_min_n() + 77
-    lens = D.lengths("zipf", n, 1.6, rng, lmax=1 << 14)
-    lens = np.insert(lens, [1, 5], 0)
-    m = lens.size
+    lens = rng.integers(1, 41, size=20_000).astype(np.int32)
+    lens = np.insert(lens, [1, 5, 9000], 0)
+    n, m = int(lens.sum()), lens.size
+    assert n >= L.cvlhip_sched_min_n()
     xi = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
     xd = rng.uniform(-1, 1, n)
     dxi, dxd = torch.from_numpy(xi).to(cv.device), torch.from_numpy(xd).to(cv.device)
@@ -171,6 +172,36 @@ def test_forced_picks_run_through_cvl_entries(cv):
     assert L.cvlhip_check_device_errors() == 0
 
 
+def test_per_segment_arms_need_short_segments_in_numbers(cv):
+    """A network that prefers the thread, wave or workgroup arm gets it only
+    where the descriptor admits it (longest segment below the arm's class
+    bound, enough segments); elsewhere the pick falls back to flat8."""
+    import torch
+    from neslgpu_amd import dispatch as D
+    L = cv.lib
+    dims = D.mlp_dims(L)
+    saved = torch.empty(dims[3], dtype=torch.float32, device=cv.device)
+    L.cvlhip_sched_get_params(saved)
+    had = L.cvlhip_sched_has_model()
+    n = 1 << 20
+    cases = {                                       # lengths -> admitted per-segment arms
+        "two halves": (D.lengths("few", n, 2, None), set()),
+        "const 16": (np.full(n // 16, 16, np.int32), {3, 4, 5}),
+        "const 100": (np.full(n // 100, 100, np.int32), {4, 5}),
+        "const 1000": (np.full(n // 1000, 1000, np.int32), {5}),
+        "const 20000": (np.full(n // 20000, 20000, np.int32), set()),
+    }
+    try:
+        for arm in (3, 4, 5):
+            L.cvlhip_sched_set_params(torch.from_numpy(forcing_params(dims, arm)).to(cv.device))
+            for name, (lens, ok) in cases.items():
+                sd = cv.segd(lens)
+                got = L.cvlhip_segd_pick(sd, 0, 4)
+                assert got == (arm if arm in ok else 0), (name, D.ARMS[arm], got)
+    finally:
+        L.cvlhip_sched_set_params(saved if had else None)
+
+
 def test_small_descriptors_carry_no_pick(cv):
     L = cv.lib
     n = L.cvlhip_sched_min_n() - 1
@@ -189,7 +220,7 @@ def test_device_pick_is_the_networks_argmin(cv):
     disp = D.Dispatcher(cv)
     L.cvlhip_sched_get_params(disp.params)
     rng = np.random.default_rng(5)
-    for dist, param in (("zipf", 1.5), ("geometric", 4), ("const", 1000), ("uniform", 2000)):
+    for dist, param in (("zipf", 1.5), ("geometric", 4), ("const", 1000), ("uniform", 2000), ("few", 3)):
         for log2n in (15, 18, 21):
             lens = D.lengths(dist, 1 << log2n, param, rng)
             sd = cv.segd(lens)
