@@ -402,7 +402,8 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
     k_scan<AddMaxD, kPairRows, true, true, PairIn, kPairBlock, true><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
-        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter, al16(s),
+        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter,
+        al16(s) && al16(scan_d),
         device_error_word());
     LAUNCHCHECK();
     if (tiles > 1) {

@@ -434,10 +434,13 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             y[k] = op(cid, v);
         }
         if constexpr (DUAL) {
+            double ya[V];
+#pragma unroll
+            for (int k = 0; k < V; k++) ya[k] = y[k].a;
+            store_lane<double, V>(in.scan, base, n, aligned, ya);
 #pragma unroll
             for (int k = 0; k < V; k++) {
                 const int64_t e = base + k;
-                if (e < n) in.scan[e] = y[k].a;
                 if ((eb[r] >> k) & 1u) {
                     const uint32_t wd = sb.heads[e >> 5];
                     const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);

@@ -24,6 +24,8 @@ Three workload sets, kept apart:
 
     python -m neslgpu_amd.dispatch train [record.json] [model.npz]
         # gather timings, train, save the model, evaluate on EVAL live
+    python -m neslgpu_amd.dispatch eval [record.json]
+        # the library's built-in model on EVAL, live
 """
 from __future__ import annotations
 
@@ -377,6 +379,20 @@ def main(argv=None):
     from .cvl import Cvl
     cv = Cvl()
     t0 = time.time()
+    if argv and argv[0] == "eval":
+        # the library's built-in network on the held-out workloads
+        if not cv.lib.cvlhip_sched_has_model():
+            raise SystemExit("libcvl_hip.so was built without a dispatcher model")
+        cv.lib.cvlhip_sched_enable(1)
+        res = {"model": "built into libcvl_hip.so from dispatch_model.npz",
+               "eval": evaluate_live(cv, EVAL_DISTS, EVAL_SIZES), "seconds": round(time.time() - t0, 1)}
+        if len(argv) > 1:
+            with open(argv[1], "w") as f:
+                json.dump(res, f, indent=1)
+        print(json.dumps({k: v for k, v in res["eval"].items() if k != "cases"}), flush=True)
+        return
+    if argv and argv[0] != "train":
+        raise SystemExit(__doc__)
     X, T, M, meta = gather(cv, TRAIN_DISTS, TRAIN_SIZES, seed=0)
     Xv, Tv, Mv, metav = gather(cv, VALID_DISTS, VALID_SIZES, seed=5)
     # the training run: the seed and length with the lowest validation regret

@@ -5,8 +5,9 @@ The HIP sources take their tile and workgroup shapes from CVLHIP_* macros
 one at a time around the current values (coordinate search: each candidate
 of one macro, every other macro at its current value):
 
-    python -m neslgpu_amd.tune build [space.json]
-        # on the build host: every variant library, compiled with -D for the
+    python -m neslgpu_amd.tune build [KEY ...]
+        # on the build host: every variant library (of the named SPACE keys,
+        # or of all), compiled with -D for the
         # one macro (only the sources that see the macro are recompiled,
         # the other objects are the default build's), into neslgpu_amd/_tune/
     python -m neslgpu_amd.tune run [out.json]
@@ -361,6 +362,9 @@ def pick(path: str) -> dict:
     defaults = variants["default"]["defaults"]
     chosen, rows = {}, []
     for macro, (values, prim) in SPACE.items():
+        if macro not in defaults:               # not searched in this run: keeps its value
+            chosen[macro] = norm(key_value(macro))
+            continue
         base = res.get(f"default:{prim}", {})
         best_v, best_ms = norm(defaults[macro]), base.get("ms") if base.get("correct") else None
         rows.append((macro, best_v, prim, base.get("ms"), base.get("correct"), "current"))
@@ -408,7 +412,8 @@ def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     cmd = argv[0] if argv else "build"
     if cmd == "build":
-        build_variants()
+        # optional: the SPACE keys to search (all by default)
+        build_variants({k: SPACE[k] for k in argv[1:]} if len(argv) > 1 else SPACE)
     elif cmd == "run":
         run_variants(argv[1] if len(argv) > 1 else os.path.join(TUNE_DIR, "results.json"))
     elif cmd == "pick":
