@@ -109,6 +109,16 @@ int cvlhip_sched_has_model(void);
  * CVLHIP_SCHED=0 in the environment sets 0 when the library loads. */
 void cvlhip_sched_enable(int on);
 
+/* ---- callers that cannot allocate -------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  add_sez (same d, s, sd, n, m and result) with its scratch in
+ * a static 4 MiB device array per GPU instead of a scratch argument; it
+ * calls no allocator.  n may be at most 1073545216: a larger n is rejected
+ * with an error before any work is queued.  Calls that overlap on one GPU
+ * race, as they share that GPU's array. */
+void cvlhip_add_sez_noalloc(vec_p d, vec_p s, vec_p sd, int n, int m);
+
 /* ---- learned dispatcher network (device pointers, fp32) -------------- */
 /* Sizes: F input features, H hidden units (two ReLU layers), A outputs (one
  * predicted log-time per schedule), P parameters in the flat array. */

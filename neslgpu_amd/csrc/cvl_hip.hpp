@@ -30,6 +30,9 @@ void usage_error(const char *fmt, ...);
 // Device word raised by kernels that met an out-of-range index (the write or
 // read is skipped instead of faulting).
 unsigned *device_error_word();
+// The error word of the no-allocation entry's static array on the current
+// device, or null before that entry's first call there.
+unsigned *noalloc_error_word();
 
 #define HIPCHECK(x)                                                        \
     do {                                                                   \

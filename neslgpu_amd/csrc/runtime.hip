@@ -202,12 +202,15 @@ int cvlhip_last_error(char *buf, int len) {
 }
 
 unsigned cvlhip_check_device_errors(void) {
-    unsigned h[1] = {0};
+    unsigned h[2] = {0, 0};
     unsigned *w = device_error_word();
+    unsigned *w2 = noalloc_error_word();        // the no-allocation entry's, once it ran here
     HIPCHECK(hipMemcpyAsync(h, w, 4, hipMemcpyDeviceToHost, g_stream));
+    if (w2) HIPCHECK(hipMemcpyAsync(h + 1, w2, 4, hipMemcpyDeviceToHost, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
     if (h[0]) HIPCHECK(hipMemsetAsync(w, 0, 4, g_stream));
-    return h[0];
+    if (h[1]) HIPCHECK(hipMemsetAsync(w2, 0, 4, g_stream));
+    return h[0] | h[1];
 }
 
 void cvlhip_synchronize(void) { HIPCHECK(hipStreamSynchronize(g_stream)); }

@@ -433,6 +433,58 @@ template CountB::T reduce_unseg<CountB>(const void *, int64_t, void *);
 template int64_t scan_scratch_bytes<AddZ>(int64_t);
 template AddZ::T reduce_unseg<AddZ>(const void *, int64_t, void *);
 
+// ------------------------------------------------------------------ no-allocation entry
+// add_sez for callers that cannot allocate (code under HIP graph capture,
+// serving loops, hosts that forbid hipMalloc): its scratch -- the look-back
+// tile counter and status words, and a device error word -- is `scratch`, a
+// static device array each GPU holds until the process ends.  Bytes
+// [0, 256) hold the error word, the scan's scratch follows.  Every call on
+// a GPU uses that GPU's one array, so calls that overlap on one GPU race.
+// The tile counter and status words are cleared by a fill kernel at the
+// start of each call; the error word starts at zero (static storage), is
+// raised only by a look-back that gave up, and is read and cleared by
+// cvlhip_check_device_errors.
+constexpr int64_t kNoAllocBytes = 4 << 20;
+constexpr int64_t kNoAllocErr = 256;        // bytes before the scan's scratch
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+static void *g_noalloc_ptr[kNoAllocDevices];   // device address of `scratch`, per device
+
+// The address of this GPU's `scratch` (looked up on the first call on each
+// device, with that device current).  The host-side name of the array is
+// not a device address; only this pointer is passed to kernels.
+static void *noalloc_scratch() {
+    int dev = -1;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) {
+        usage_error("cvlhip_add_sez_noalloc: device %d is beyond the %d devices it serves", dev, kNoAllocDevices);
+        return nullptr;
+    }
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+            return nullptr;
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    return g_noalloc_ptr[dev];
+}
+
+unsigned *noalloc_error_word() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return nullptr;
+    return (unsigned *)g_noalloc_ptr[dev];
+}
+
+// Bytes of `scratch` add_sez needs for n elements.
+static int64_t noalloc_need(int64_t n) {
+    const int64_t a = scan_scratch_bytes<AddZ>(n), b = sched_scratch_bytes<AddZ, false>(n);
+    return kNoAllocErr + (a > b ? a : b);
+}
+
 }  // namespace cvlhip
 
 using namespace cvlhip;
@@ -467,6 +519,20 @@ using namespace cvlhip;
     unsigned int OPN##_re##TY##_inplace(void) { return INPLACE_NONE; }
 
 extern "C" {
+// add_sez without a scratch argument, allocating nothing (see above).  The
+// scratch of n elements is 768 + 8 ceil(n / 2048) bytes (1-row look-back
+// tiles of the dispatched schedule), so n may be at most 1,073,545,216.
+void cvlhip_add_sez_noalloc(vec_p d, vec_p s, vec_p sd, int n, int m) {
+    if (noalloc_need(n) > kNoAllocBytes) {
+        usage_error("cvlhip_add_sez_noalloc: %d elements need %lld bytes of scratch, more than its %lld", n,
+                    (long long)noalloc_need(n), (long long)kNoAllocBytes);
+        return;
+    }
+    char *p = (char *)noalloc_scratch();
+    if (!p) return;
+    scan_seg<AddZ>(d, s, sd, n, m, p + kNoAllocErr, -1, (unsigned *)p);
+}
+
 void cvlhip_add_sed_max_red(vec_p scan_d, vec_p red_d, vec_p s, vec_p sd, int n, int m, vec_p scratch) {
     add_max_pair((double *)scan_d, (double *)red_d, (const double *)s, sd, n, m, scratch);
 }
