@@ -32,6 +32,7 @@
 
 #include "../../include/vcode_fused.h"
 
+#include <algorithm>
 #include <array>
 #include <cerrno>
 #include <cstdarg>
@@ -54,6 +55,8 @@ static_assert(sizeof(CvlTriple) == sizeof(cvl_triple_t), "cvl_triple_t layout");
 static_assert(T_Segdes == (int)Segdes && T_Given == (int)Given, "TYPE values");
 static_assert(D_Scan == (int)Scan && D_NotSupported == (int)NotSupported, "cvl_desc_t values");
 
+static bool g_abort_on_error = false;      // -d abort
+
 [[noreturn]] static void die(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -62,6 +65,7 @@ static_assert(D_Scan == (int)Scan && D_NotSupported == (int)NotSupported, "cvl_d
     std::fprintf(stderr, "\n");
     va_end(ap);
     std::fflush(stdout);
+    if (g_abort_on_error) std::abort();
     std::exit(1);
 }
 
@@ -203,6 +207,9 @@ class Arena {
     vec_p addr(size_t off) const { return base_ + off * 8; }
     size_t peak() const { return peak_; }
     size_t total() const { return total_; }
+    size_t used() const { return used_; }
+    size_t free_blocks() const { return by_off_.size(); }
+    size_t largest_free() const { return by_size_.empty() ? 0 : by_size_.rbegin()->first; }
 
   private:
     void insert(size_t off, size_t len) {
@@ -270,6 +277,7 @@ class Machine {
     Machine(const Program &prog, const Options &opt) : prog_(prog), opt_(opt), sigs_(prog.code.size()) {}
 
     int run() {
+        g_abort_on_error = opt_.abort_on_error;
         be_.open(opt_.cvl_library);
         if (be_.CVL_init) {
             static char empty[1] = {0};
@@ -291,7 +299,9 @@ class Machine {
         if (it == prog_.labels.end()) die("start function %s not found", prog_.entry.c_str());
         alignas(64) unsigned char t0[64], t1[64];
         if (opt_.time_run) be_.tgt_fos(t0);
+        if (!opt_.profile_path.empty()) profile_init();
         execute(it->second);
+        if (!opt_.profile_path.empty()) profile_report();
         if (opt_.time_run) {
             be_.tgt_fos(t1);
             std::fprintf(stderr, "elapsed time = %f seconds\n", be_.tdf_fos(t1, t0));
@@ -1232,9 +1242,137 @@ class Machine {
     }
 
     // ---------------------------------------------------------- main loop
+    // ---------------------------------------------------------- diagnostics
+    // -d runtime, -d stack, -d values, -d heap (nesl/vcode/main.c:130-139)
+    // and the per-function profile the reference builds with VCODE_PROFILE
+    // (vcode-profile.c:79-136): one line per function, sorted by decreasing
+    // exclusive time.  A function's time runs from its CALL to its RET; the
+    // library's stream is synchronized at both, so device work is charged
+    // to the function that queued it.
+    struct FuncTime {
+        std::string name;
+        long long calls = 0;
+        double time = 0, min = 1e9, max = 0, cum = 0;
+    };
+    struct TimeFrame {
+        int func;
+        double start, child;
+    };
+
+    void profile_init() {
+        sync_ = reinterpret_cast<void (*)()>(be_.sym("cvlhip_synchronize"));
+        for (const auto &l : prog_.labels) {
+            func_of_pc_[l.second] = (int)funcs_.size();
+            funcs_.push_back(FuncTime{l.first});
+        }
+    }
+    double now() {
+        if (sync_) sync_();
+        timespec t;
+        clock_gettime(CLOCK_REALTIME, &t);
+        return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+    }
+    void profile_call(int target_pc) {
+        auto it = func_of_pc_.find(target_pc);
+        times_.push_back(TimeFrame{it == func_of_pc_.end() ? -1 : it->second, now(), 0.0});
+    }
+    void profile_return() {
+        if (times_.empty()) return;
+        const TimeFrame top = times_.back();
+        times_.pop_back();
+        const double dt = now() - top.start;
+        if (!times_.empty()) times_.back().child += dt;
+        if (top.func < 0) return;
+        FuncTime &f = funcs_[top.func];
+        f.cum += dt;
+        const double self = dt - top.child;
+        f.time += self;
+        if (self < f.min) f.min = self;
+        if (self > f.max) f.max = self;
+        f.calls++;
+    }
+    void profile_report() {
+        std::vector<int> order(funcs_.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            const FuncTime &p = funcs_[a], &q = funcs_[b];
+            if ((p.calls > 0) != (q.calls > 0)) return p.calls > 0;
+            if (p.calls > 0 && p.time != q.time) return p.time > q.time;
+            return p.name < q.name;
+        });
+        FILE *out = std::fopen(opt_.profile_path.c_str(), "w");
+        if (!out) {
+            std::fprintf(stderr, "unable to open \"%s\"\n", opt_.profile_path.c_str());
+            return;
+        }
+        std::fprintf(stderr, "dumping profile information to %s\n", opt_.profile_path.c_str());
+        std::fprintf(out, "# Calls     Time         Min.        Avg.        Max.    Cumulative   Name\n");
+        std::fprintf(out, "-------  ----------  ----------  ----------  ----------  ----------   ---------------------\n");
+        for (int i : order) {
+            const FuncTime &p = funcs_[i];
+            if (p.calls > 0)
+                std::fprintf(out, " %6lld  %10.6f  %10.6f  %10.6f  %10.6f  %10.6f   %s\n", p.calls, p.time, p.min,
+                             p.time / (double)p.calls, p.max, p.cum, p.name.c_str());
+            else
+                std::fprintf(out, "      0     ----        ----        ----        ----        ----      %s\n",
+                             p.name.c_str());
+        }
+        std::fclose(out);
+    }
+
+    // One stack entry: type and length, and with -d values its first values.
+    void trace_entry(const Vec *v) {
+        if (v->type == T_Pair) {
+            std::fprintf(stderr, " <pair>");
+            return;
+        }
+        if (v->type == T_Segdes) {
+            std::fprintf(stderr, " SEGDES[%d segments, %d elements]", v->len, v->seg_len);
+            return;
+        }
+        std::fprintf(stderr, " %s[%d]", type_name(v->type), v->len);
+        if (!opt_.value_trace || v->len <= 0) return;
+        const int n = v->len < 8 ? v->len : 8;
+        std::fprintf(stderr, "(");
+        if (v->type == T_Float) {
+            double h[8];
+            Scratch s(*this, be_.v2c_fud_scratch(n));
+            be_.v2c_fud(h, v->p, n, s.p);
+            for (int i = 0; i < n; ++i) std::fprintf(stderr, " %.6g", h[i]);
+        } else if (v->type == T_Bool) {
+            cvl_bool h[8];
+            Scratch s(*this, be_.v2c_fub_scratch(n));
+            be_.v2c_fub(h, v->p, n, s.p);
+            for (int i = 0; i < n; ++i) std::fprintf(stderr, " %c", h[i] ? 'T' : 'F');
+        } else {
+            int h[8];
+            Scratch s(*this, be_.v2c_fuz_scratch(n));
+            be_.v2c_fuz(h, v->p, n, s.p);
+            for (int i = 0; i < n; ++i) std::fprintf(stderr, " %d", h[i]);
+        }
+        std::fprintf(stderr, "%s )", v->len > n ? " ..." : "");
+    }
+    void trace_after(const Instr &in) {
+        if (opt_.stack_trace || opt_.value_trace) {
+            std::fprintf(stderr, "stack after line %d (%zu):", in.line, st_.size());
+            for (const Vec *v : st_) trace_entry(v);
+            std::fprintf(stderr, "\n");
+        }
+        if (opt_.heap_trace)
+            std::fprintf(stderr, "heap after line %d: %zu of %zu units in use, %zu free blocks, largest %zu\n",
+                         in.line, arena_.used(), arena_.total(), arena_.free_blocks(), arena_.largest_free());
+    }
+
     void execute(int pc) {
         std::vector<int> ret;
         ret.push_back(-1);
+        std::vector<const std::string *> frames;             // -d runtime
+        if (opt_.runtime_trace) {
+            std::fprintf(stderr, "call %s\n", prog_.entry.c_str());
+            frames.push_back(&prog_.entry);
+        }
+        const bool profiling = !opt_.profile_path.empty();
+        if (profiling) profile_call(pc);
         alignas(64) unsigned char t_start[64] = {0}, t_end[64];
         while (pc >= 0) {
             const Instr &in = prog_.code[pc];
@@ -1280,10 +1418,22 @@ class Machine {
                     break;
                 }
                 case OP_CALL:
+                    if (opt_.runtime_trace) {
+                        std::fprintf(stderr, "%*scall %s (line %d)\n", 2 * (int)frames.size(), "",
+                                     in.label.c_str(), in.line);
+                        frames.push_back(&in.label);
+                    }
+                    if (profiling) profile_call(in.branch);
                     ret.push_back(pc + 1);
                     next = in.branch;
                     break;
                 case OP_RET:
+                    if (opt_.runtime_trace && !frames.empty()) {
+                        const std::string *f = frames.back();
+                        frames.pop_back();
+                        std::fprintf(stderr, "%*sreturn from %s\n", 2 * (int)frames.size(), "", f->c_str());
+                    }
+                    if (profiling) profile_return();
                     next = ret.back();
                     ret.pop_back();
                     break;
@@ -1341,6 +1491,7 @@ class Machine {
                 case OP_PACK: pack(pc, in); break;
                 default: vector_op(pc, in); break;
             }
+            if (opt_.stack_trace || opt_.value_trace || opt_.heap_trace) trace_after(in);
             pc = next;
         }
         if (rec_.on) flush();
@@ -1370,6 +1521,11 @@ class Machine {
     void (*graph_destroy_)(void *) = nullptr;
     bool io_error_ = false;
     std::string io_msg_;
+    // -p
+    void (*sync_)() = nullptr;
+    std::map<int, int> func_of_pc_;
+    std::vector<FuncTime> funcs_;
+    std::vector<TimeFrame> times_;
 };
 
 int run(const Program &prog, const Options &opt) {

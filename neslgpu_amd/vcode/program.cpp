@@ -294,7 +294,10 @@ class Lexer {
 // ------------------------------------------------------------------ parser
 class Parser {
   public:
-    Parser(const std::string &text, Program &p) : lex_(text), prog_(p) { advance(); }
+    Parser(const std::string &text, Program &p, bool lex_trace, bool link_trace)
+        : lex_(text), prog_(p), lex_trace_(lex_trace), link_trace_(link_trace) {
+        advance();
+    }
 
     bool parse() {
         if (tok_.kind == TK_EOF) return fail("empty program");
@@ -305,7 +308,13 @@ class Parser {
     }
 
   private:
-    void advance() { tok_ = lex_.next(); }
+    void advance() {
+        tok_ = lex_.next();
+        if (lex_trace_) {
+            static const char *kind[] = {"EOF", "INT", "REAL", "STRING", "(", ")", "I", "O", "WORD"};
+            std::fprintf(stderr, "lex: line %d %s %s\n", tok_.line, kind[tok_.kind], tok_.text.c_str());
+        }
+    }
 
     bool fail(const std::string &msg) {
         std::fprintf(stderr, "vinterp: line %d: parse error: %s near \"%s\"\n", tok_.line, msg.c_str(),
@@ -552,6 +561,9 @@ class Parser {
                 ok = false;
             } else {
                 in.branch = it->second;
+                if (link_trace_)
+                    std::fprintf(stderr, "link: line %d: CALL %s -> instruction %d\n", in.line, in.label.c_str(),
+                                 in.branch);
             }
         }
         if (prog_.entry.empty()) {
@@ -563,14 +575,40 @@ class Parser {
 
     Lexer lex_;
     Program &prog_;
+    bool lex_trace_, link_trace_;
     Token tok_;
     std::vector<int> ifs_;
     bool main_seen_ = false;
 };
 
-bool parse_program(const std::string &text, Program &prog) {
-    Parser p(text, prog);
+bool parse_program(const std::string &text, Program &prog, bool lex_trace, bool link_trace) {
+    Parser p(text, prog, lex_trace, link_trace);
     return p.parse();
+}
+
+void show_program(const Program &prog, FILE *out) {
+    std::map<int, std::string> funcs;
+    for (const auto &l : prog.labels) funcs[l.second] = l.first;
+    for (size_t pc = 0; pc < prog.code.size(); ++pc) {
+        auto f = funcs.find((int)pc);
+        if (f != funcs.end()) std::fprintf(out, "FUNC %s\n", f->second.c_str());
+        const Instr &in = prog.code[pc];
+        std::fprintf(out, "%6zu  line %-6d %s", pc, in.line, op_info(in.op).name);
+        if (in.type != T_Illegal && in.type != T_None) std::fprintf(out, " %s", type_name(in.type));
+        switch (in.op) {
+            case OP_COPY: case OP_POP: case OP_CPOP: std::fprintf(out, " %d %d", in.a, in.b); break;
+            case OP_FUSED: std::fprintf(out, " %d", in.a); break;
+            case OP_CALL: std::fprintf(out, " %s -> %d", in.label.c_str(), in.branch); break;
+            case OP_IF: case OP_ELSE: std::fprintf(out, " -> %d", in.branch); break;
+            case OP_CONST:
+                if (in.pooled) std::fprintf(out, " <constant %d>", in.pool_index);
+                else if (in.type == T_Float) std::fprintf(out, " %.17g", in.dval);
+                else std::fprintf(out, " %lld", in.ival);
+                break;
+            default: break;
+        }
+        std::fprintf(out, "\n");
+    }
 }
 
 }  // namespace vcode

@@ -149,15 +149,27 @@ struct Program {
 };
 
 // Parse and link a VCODE source; returns false (message on stderr) on error.
-bool parse_program(const std::string &text, Program &prog);
+// lex_trace prints every token, link_trace every resolved call
+// (-d lex, -d link; nesl/vcode/main.c:194-214).
+bool parse_program(const std::string &text, Program &prog, bool lex_trace = false, bool link_trace = false);
+
+// The linked program, one instruction per line with its index (-d program).
+void show_program(const Program &prog, FILE *out);
 
 struct Options {
     std::string cvl_library;     // CVL backend shared library
     std::string fused_library;   // -l
     long long arena_units = 0;   // -m (0: default)
     bool time_run = false;       // -t
-    bool trace = false;          // -d command
+    bool trace = false;          // -d command: every instruction executed
+    bool runtime_trace = false;  // -d runtime: the call stack
+    bool stack_trace = false;    // -d stack: the vector stack after each instruction
+    bool value_trace = false;    // -d values: and the first values of each entry
+    bool heap_trace = false;     // -d heap: arena use after each instruction
+    bool abort_on_error = false; // -d abort: abort() (core dump) instead of exit on an error
+    bool garbage_notify = false; // -g: accepted; this arena never compacts
     bool graphs = false;         // -G: replay repeated basic blocks as HIP graphs
+    std::string profile_path;    // -p: per-function times written here (.vprof)
 };
 
 int run(const Program &prog, const Options &opt);

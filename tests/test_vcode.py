@@ -95,3 +95,63 @@ def test_serial_reproduces_golden(name):
     assert stable(r.stdout, rec["volatile_stdout_lines"]) == stable(want, rec["volatile_stdout_lines"])
     err = "".join(l for l in r.stderr.splitlines(keepends=True) if not l.startswith("rnd_wuz:"))
     assert err == open(os.path.join(GOLD, name + ".err")).read()
+
+
+def _serial_or_skip():
+    if not os.path.exists(SERIAL):
+        pytest.skip("reference serial CVL not built (make -C oracle)")
+
+
+def test_diagnostic_switches(tmp_path):
+    """The reference interpreter's debugging switches (nesl/vcode/main.c:
+    130-139, 194-214) on quicksort of 1000 keys over the serial CVL: each
+    trace prints what it names and the program's own output is unchanged;
+    -p writes <program>.vprof in the reference's format
+    (vcode-profile.c:104-131) with one line per function."""
+    _serial_or_skip()
+    src = tmp_path / "qs.1000.vcode"
+    src.write_text(open(os.path.join(PROGS, "qs.1000.vcode")).read())
+    want = open(os.path.join(GOLD, "qs.1000.vcode.out")).read()
+
+    def run(*flags):
+        r = subprocess.run([V.VINTERP, "-B", SERIAL, *flags, str(src)], capture_output=True, text=True,
+                           timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr[-500:]
+        vol = MANIFEST["qs.1000.vcode"]["volatile_stdout_lines"]
+        keep = lambda t: [l for i, l in enumerate(t.splitlines()) if i not in vol]
+        assert keep(r.stdout) == keep(want), flags
+        return r.stderr
+
+    prog = V.read(str(src))
+    funcs = [l.split()[1] for l in prog.splitlines() if l.startswith("FUNC ")]
+    err = run("-d", "program")
+    assert sorted(l.split()[1] for l in err.splitlines() if l.startswith("FUNC ")) == sorted(funcs)
+    err = run("-d", "link")
+    assert sum(l.startswith("link: ") for l in err.splitlines()) == prog.count("\nCALL ")
+    err = run("-d", "lex")
+    assert "lex: line 3 WORD FUNC" in err
+    err = run("-d", "runtime")
+    assert "call QUICKSORT_80" in err and "return from QUICKSORT_80" in err
+    err = run("-d", "stack", "-d", "values", "-d", "heap")
+    assert "stack after line 4 (1): INT[1]( 1000 )" in err and "heap after line 4:" in err
+    run("-p", "-g", "-c")
+    lines = (tmp_path / "qs.1000.vprof").read_text().splitlines()
+    assert lines[0].split() == ["#", "Calls", "Time", "Min.", "Avg.", "Max.", "Cumulative", "Name"]
+    rows = {l.split()[-1]: l.split() for l in lines[2:]}
+    assert set(rows) == set(funcs)
+    # calls counted; the entry's cumulative time holds its (non-recursive) callee's
+    assert int(rows["QUICKSORT_80"][0]) > 1 and int(rows[funcs[0]][0]) == 1
+    assert float(rows[funcs[0]][5]) >= float(rows["QS_4"][5])
+
+
+def test_abort_switch(tmp_path):
+    """-d abort turns an interpreter error into abort() (a core dump where
+    the system allows one) instead of exit(1)."""
+    _serial_or_skip()
+    src = tmp_path / "bad.vcode"
+    src.write_text("FUNC MAIN1_1\nPOP 1 0\nRET\n")
+    r = subprocess.run([V.VINTERP, "-B", SERIAL, str(src)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "vinterp:" in r.stderr
+    r = subprocess.run(["bash", "-c", f"ulimit -c 0; exec {V.VINTERP} -B {SERIAL} -d abort {src}"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode in (-6, 128 + 6), r.returncode      # SIGABRT
