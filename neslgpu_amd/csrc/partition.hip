@@ -230,10 +230,11 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
     __shared__ unsigned char s_dig[PACKED ? 1 : kPartTile];
 
     const int tid = threadIdx.x;
+    constexpr bool LB = FIRST && kPartLookback;    // compiled in only when selected
     for (int i = tid; i < kMaxFan; i += kPartThreads) s_hist[i] = 0;
-    if (FIRST && lb && tid == 0) s_bt = atomicAdd(lb_ctr, 1u);
+    if (LB && tid == 0) s_bt = atomicAdd(lb_ctr, 1u);
     __syncthreads();
-    const int64_t bt = (FIRST && lb) ? (int64_t)s_bt : xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t bt = LB ? (int64_t)s_bt : xcd_tile(blockIdx.x, gridDim.x);
     const int64_t region = bt / tiles_per_region;
     const int64_t chunk = bt % tiles_per_region;
     const int64_t rlo = region << plog;
@@ -322,10 +323,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
             s_start[d] = run;
             run += c[i];
             if (tile_ofs) s_gofs[d] = tile_ofs[bt * kMaxFan + d];
-            else if (!(FIRST && lb)) s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
+            else if (!LB) s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
         }
     }
-    if (FIRST && lb && tid < kMaxFan) {
+    if (LB && tid < kMaxFan) {
         // one thread per child: the counts of the tiles before this one
         const int d = tid;
         const unsigned c = s_hist[d];

@@ -257,9 +257,9 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     for (int i = tid; i < W * kRadix; i += kRankThreads) (&s_cnt[0][0])[i] = 0;
     // a chained pass numbers its tiles in the order they start; the
     // order of the tiles' runs within a digit is then their data order
-    if (a.chain && tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+    if (kRankChain && a.chain && tid == 0) s_tile = atomicAdd(a.ticket, 1u);
     __syncthreads();
-    const int64_t tile = a.chain ? (int64_t)s_tile : xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t tile = (kRankChain && a.chain) ? (int64_t)s_tile : xcd_tile(blockIdx.x, gridDim.x);
     const int64_t t0 = tile * kRankTile;
     const int64_t count = a.n - t0 < kRankTile ? a.n - t0 : kRankTile;
     const int64_t wb = t0 + (int64_t)w * kWaveSpan;
@@ -357,8 +357,9 @@ loaded:
             run += c;
         }
         s_tot[d] = run;
-        const unsigned prefix = a.chain ? chain_exclusive(a.chain, tile, kRadix, d, run, a.parity, a.err)
-                                        : a.tile_ofs[tile * kRadix + d];
+        const unsigned prefix = (kRankChain && a.chain)
+                                    ? chain_exclusive(a.chain, tile, kRadix, d, run, a.parity, a.err)
+                                    : a.tile_ofs[tile * kRadix + d];
         s_gbase[d] = a.starts[d] + prefix;
     }
     __syncthreads();

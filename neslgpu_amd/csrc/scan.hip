@@ -17,6 +17,7 @@
 #include "scan.hpp"
 #include "scan_kernel.hpp"
 #include "segsched.hpp"
+#include "pairpass.hpp"
 #include "../../include/cvl.h"
 
 namespace cvlhip {
@@ -361,32 +362,18 @@ void reduce_seg(void *d, const void *s, const void *sd, int64_t n, int64_t m, vo
     LAUNCHCHECK();
 }
 
-// add_sed and max_red of the same values in one pass (one read of s): the
-// scan runs on component a and the reduction on component b of PairD.
-// Rows per wave of its tiles: a pair is four registers per element, so 8
-// rows keep the occupancy of the 16-row double scan.
-#ifndef CVLHIP_PAIR_ROWS
-#define CVLHIP_PAIR_ROWS 8
-#endif
-constexpr int kPairRows = CVLHIP_PAIR_ROWS;
-using AddMaxD = PairOp<AddD, MaxD>;
-
-#ifndef CVLHIP_PAIR_BLOCK
-#define CVLHIP_PAIR_BLOCK 256
-#endif
-constexpr int kPairBlock = CVLHIP_PAIR_BLOCK;
-
-// Counts the tiles of the shape the pass is launched with: the scan's
-// status words, then the reduction's records.
+// add_sed and max_red of the same values in one pass (one read of s):
+// k_pair_scan_max (pairpass.hpp), then the maximum's records of segments
+// that cross tiles.
 int64_t add_max_pair_scratch_bytes(int64_t n) {
-    using S = ScanShape<AddMaxD, kPairRows, kPairBlock>;
+    using S = PairShape<kPairRows, kPairBlock>;
     const int64_t tiles = (n + S::TILE - 1) / S::TILE;
-    return kScanHeaderBytes + split_status_bytes<AddMaxD>(tiles) + NolbRecords<double>::bytes(tiles) + 256;
+    return kScanHeaderBytes + pair_status_bytes(tiles) + NolbRecords<double>::bytes(tiles) + 256;
 }
 
 void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd, int64_t n, int64_t m,
                   void *scratch) {
-    using S = ScanShape<AddMaxD, kPairRows, kPairBlock>;
+    using S = PairShape<kPairRows, kPairBlock>;
     if (m <= 0) return;
     if (!scratch) usage_error("add_sed/max_red pair called without scratch");
     SegdView v(sd, n, m);
@@ -398,16 +385,15 @@ void add_max_pair(double *scan_d, double *red_d, const double *s, const void *sd
     const int64_t tiles = (n + S::TILE - 1) / S::TILE;
     if (tiles > 0x7fffffff) usage_error("vector too long");
     // the scan's status words are cleared (the records are all written)
-    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + split_status_bytes<AddMaxD>(tiles)) / 4);
+    if (tiles > 1) fill_u32(scratch, 0, (kScanHeaderBytes + pair_status_bytes(tiles)) / 4);
     unsigned *counter = tiles > 1 ? (unsigned *)scratch : nullptr;
     uint64_t *status = (uint64_t *)((char *)scratch + kScanHeaderBytes);
-    k_scan<AddMaxD, kPairRows, true, true, PairIn, kPairBlock, true><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
-        PairIn{s, scan_d, red_d}, nullptr, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter,
-        al16(s) && al16(scan_d),
-        device_error_word());
+    k_pair_scan_max<kPairRows, kPairBlock><<<(unsigned)tiles, kPairBlock, 0, g_stream>>>(
+        s, scan_d, red_d, n, SegBits{v.heads, v.wcnt, v.nz, nullptr, m}, status, counter,
+        al16(s) && al16(scan_d), device_error_word());
     LAUNCHCHECK();
     if (tiles > 1) {
-        k_seg_reduce_fix<MaxD><<<1, kFixThreads, 0, g_stream>>>(split_records<AddMaxD>(status, tiles), tiles, red_d);
+        k_seg_reduce_fix<MaxD><<<1, kFixThreads, 0, g_stream>>>(pair_records(status, tiles), tiles, red_d);
         LAUNCHCHECK();
     }
 }

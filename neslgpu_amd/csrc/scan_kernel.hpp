@@ -106,66 +106,6 @@ struct ArrayIn {
     }
 };
 
-// A loader that also says where results go when one pass produces two of
-// them (io_dual): the kernel then writes the exclusive scan of component a
-// and the per-segment reduction of component b itself.
-template <class In> struct io_dual { static constexpr bool value = false; };
-
-// Two double results of the same values in one pass: component a runs a
-// segmented scan with A, component b a segmented reduction with B.
-struct PairD {
-    double a, b;
-};
-template <class A, class B>
-struct PairOp {
-    using T = PairD;
-    using OpA = A;
-    using OpB = B;
-    __host__ __device__ static T neutral() { return {A::neutral(), B::neutral()}; }
-    __host__ __device__ static T cvl_id() { return {A::cvl_id(), B::cvl_id()}; }
-    __host__ __device__ T operator()(T x, T y) const { return {A()(x.a, y.a), B()(x.b, y.b)}; }
-};
-template <class A, class B>
-struct seq_combine<PairOp<A, B>> {
-    static constexpr bool value = seq_combine<A>::value || seq_combine<B>::value;
-};
-// Two pairs per lane and row: one 16-byte load of two doubles.
-#ifndef CVLHIP_PAIR_V
-#define CVLHIP_PAIR_V 2
-#endif
-template <class A, class B>
-struct lane_values<PairOp<A, B>> { static constexpr int value = CVLHIP_PAIR_V; };
-
-// Reads a double vector as pairs (x, x) and stores the scan of a into
-// `scan` and the reduction of b into `red` (one value per segment).
-struct PairIn {
-    const double *p;
-    double *scan;
-    double *red;
-    template <class Op, int V>
-    __device__ __forceinline__ void load(int64_t base, int64_t n, bool aligned, PairD (&x)[V]) const {
-        if constexpr (V == 2) {
-            if (aligned && base + 2 <= n) {
-                const typename Vec16<double>::type w = *(const typename Vec16<double>::type *)(p + base);
-                const double w0 = w[0], w1 = w[1];
-                x[0] = {w0, w0};
-                x[1] = {w1, w1};
-                return;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            if (base + k < n) {
-                const double v = p[base + k];
-                x[k] = {v, v};
-            } else {
-                x[k] = Op::neutral();
-            }
-        }
-    }
-};
-template <> struct io_dual<PairIn> { static constexpr bool value = true; };
-
 // Per-tile records of a segmented reduction run without look-back (NOLB):
 // each tile reduces with an empty incoming prefix and records
 //   agg[t], flag[t]: its aggregate under the segmented operator (flag: the
@@ -198,30 +138,15 @@ struct NolbRecords {
 };
 
 // One kernel for: unsegmented exclusive scan (SEG=false, REDUCE=false),
-// segmented exclusive scan (SEG=true, REDUCE=false), segmented reduce
-// (SEG=true, REDUCE=true; `out` has one element per segment) and, with a
-// dual loader (SEG=true, REDUCE=true), both at once on a pair.
+// segmented exclusive scan (SEG=true, REDUCE=false) and segmented reduce
+// (SEG=true, REDUCE=true; `out` has one element per segment).
 // NOLB (segmented reductions only): no look-back; `status` holds the
 // NolbRecords of gridDim.x tiles and k_seg_reduce_fix finishes the
-// segments that cross tiles.  A dual pass with NOLB splits the pair: the
-// scan (component a) chains its tiles with a look-back on its own, 8-byte
-// values (status words first, split_records() after them), and the
-// reduction (component b) runs without look-back, finished by
-// k_seg_reduce_fix<OpB> on its records.
+// segments that cross tiles.
 //
 // scan_tile is the work of one workgroup on one tile (`tile` of `ntiles`;
 // NOLB records are laid out for ntiles tiles).  k_scan runs one tile per
 // workgroup; the segmented-op schedules (k_seg_sched) loop over tiles.
-// Where a split dual pass keeps the reduction's records: after the status
-// words of its ntiles tiles.
-template <class Op>
-__host__ __device__ inline int64_t split_status_bytes(int64_t ntiles) {
-    return (ntiles * TileStatus<double>::kWords * 8 + 255) & ~(int64_t)255;
-}
-template <class Op>
-__host__ __device__ inline NolbRecords<double> split_records(void *status, int64_t ntiles) {
-    return NolbRecords<double>::at((char *)status + split_status_bytes<Op>(ntiles), ntiles);
-}
 
 template <class Op, int R, bool SEG, bool REDUCE, class In, int B, bool NOLB>
 __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
@@ -231,9 +156,6 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
     using S = ScanShape<Op, R, B>;
     constexpr int V = S::V;
     constexpr uint32_t VMASK = (1u << V) - 1u;
-    constexpr bool DUAL = io_dual<In>::value;
-    constexpr bool SPLIT = DUAL && NOLB;     // look-back on component a only
-    static_assert(!DUAL || (SEG && REDUCE), "a dual pass is a segmented scan and reduction");
     static_assert(!NOLB || (SEG && REDUCE), "only segmented reductions run without look-back");
     const Op op;
 
@@ -270,18 +192,36 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
     // bits 16.. of eb[r]) + the part's own heads up to k.  With no empty
     // segments the output slot of a segment is that count - 1 and nz need
     // not be read.
-    // (A dual pass reads wc at the stores instead: its registers are at the
-    // edge of three waves per SIMD.)
-    constexpr bool WC = REDUCE && !DUAL;
+    constexpr bool WC = REDUCE;
     int wc[WC ? R : 1];
     bool nz_ident = false;
     if constexpr (WC) {
         if (sb.m > 0) nz_ident = sb.wcnt[(n + 31) >> 5] == sb.m;
     }
 #pragma unroll
+    for (int r = 0; r < R; r++) in.template load<Op, V>(wbase + (int64_t)r * S::ROW, n, aligned, x[r]);
+    // The tile's head bitmap words (and, for reductions, head counts) are
+    // staged in LDS with one coalesced pass, so the rows below read them
+    // there instead of with two more vector loads each.  Words past the
+    // descriptor's last (index (n+31)/32) read as 0.
+    constexpr int HW = (int)(S::TILE / 32) + 2;
+    __shared__ uint32_t s_heads[SEG ? HW : 1];
+    __shared__ int s_wcnt[WC ? HW : 1];
+    const int64_t w0 = (tile * S::TILE) >> 5;
+    if constexpr (SEG) {
+        const int64_t W = (n + 31) >> 5;
+        __syncthreads();                    // a caller looping over tiles is done with the last one's
+        for (int i = threadIdx.x; i < HW; i += B) {
+            const int64_t wd = w0 + i;
+            s_heads[i] = wd <= W ? sb.heads[wd] : 0u;
+            if constexpr (WC) s_wcnt[i] = wd <= W ? sb.wcnt[wd] : 0;
+        }
+        __syncthreads();
+    }
+    auto head_word = [&](int64_t wd) -> uint32_t { return s_heads[wd - w0]; };
+#pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t base = wbase + (int64_t)r * S::ROW;
-        in.template load<Op, V>(base, n, aligned, x[r]);
         hb[r] = 0;
         eb[r] = 0;
         if constexpr (WC) wc[r] = 0;
@@ -289,7 +229,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             if (base < n) {
                 const int64_t wd = base >> 5;
                 const int sh = (int)(base & 31);
-                const uint64_t two = (uint64_t)sb.heads[wd] | ((uint64_t)sb.heads[wd + 1] << 32);
+                const uint64_t two = (uint64_t)head_word(wd) | ((uint64_t)head_word(wd + 1) << 32);
                 hb[r] = (uint32_t)(two >> sh) & VMASK;
                 if constexpr (REDUCE) {
                     uint32_t nxt = (uint32_t)(two >> (sh + 1)) & VMASK;   // head at element + 1
@@ -301,7 +241,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
                     eb[r] = nxt;
                     if constexpr (WC) {
                         eb[r] |= (uint32_t)__popc((uint32_t)two & ((1u << sh) - 1u)) << 16;
-                        if (nxt) wc[r] = sb.wcnt[wd];
+                        if (nxt) wc[r] = s_wcnt[wd - w0];
                     }
                 }
             }
@@ -322,32 +262,40 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             if (SEG && ((hb[r] >> k) & 1u)) { a = x[r][k]; af = 1; }
             else a = op(a, x[r][k]);
         }
-        if constexpr (SEG) seg_wave_scan(af, a, op);
-        else a = wave_inclusive_scan(a, op);
-        T ea = shfl_up1(a);
-        int ef = SEG ? shfl_up1(af) : 0;
-        if (lane == 0) { ea = Op::neutral(); ef = 0; }
+        // A row of the wave with no segment head (most rows of long
+        // segments) scans without the flags: the same steps and bits.
+        if constexpr (SEG) {
+            if (ballot(hb[r] != 0u)) seg_wave_scan(af, a, op);
+            else a = wave_inclusive_scan(a, op);
+        } else {
+            a = wave_inclusive_scan(a, op);
+        }
         const T ra = bcast_last(a);
         const int rf = SEG ? bcast_last(af) : 0;
-        T p = ef ? ea : op(carry, ea);
-        int pf = cf | ef;
+        // A reduction needs per-element values only where a segment ends:
+        // a row of the wave with no end only moves the carry on.
+        bool elems = true;
+        if constexpr (REDUCE) elems = ballot((eb[r] & VMASK) != 0u) != 0;
+        if (elems) {
+            T ea = shfl_up1(a);
+            int ef = SEG ? shfl_up1(af) : 0;
+            if (lane == 0) { ea = Op::neutral(); ef = 0; }
+            T p = ef ? ea : op(carry, ea);
+            int pf = cf | ef;
 #pragma unroll
-        for (int k = 0; k < V; k++) {
-            const bool h = SEG && ((hb[r] >> k) & 1u);
-            T y;
-            if constexpr (DUAL) {             // a: exclusive, b: inclusive; same `needs`
-                if (h) { y.a = Op::neutral().a; p = x[r][k]; pf = 1; }
-                else { y.a = p.a; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
-                y.b = p.b;
-            } else if constexpr (!REDUCE) {
-                if (h) { y = Op::neutral(); p = x[r][k]; pf = 1; }
-                else { y = p; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
-            } else {
-                if (h) { p = x[r][k]; pf = 1; }
-                else { p = op(p, x[r][k]); if (!pf) needs |= 1ull << (r * V + k); }
-                y = p;
+            for (int k = 0; k < V; k++) {
+                const bool h = SEG && ((hb[r] >> k) & 1u);
+                T y;
+                if constexpr (!REDUCE) {
+                    if (h) { y = Op::neutral(); p = x[r][k]; pf = 1; }
+                    else { y = p; if (!pf) needs |= 1ull << (r * V + k); p = op(p, x[r][k]); }
+                } else {
+                    if (h) { p = x[r][k]; pf = 1; }
+                    else { p = op(p, x[r][k]); if (!pf) needs |= 1ull << (r * V + k); }
+                    y = p;
+                }
+                x[r][k] = y;
             }
-            x[r][k] = y;
         }
         carry = rf ? ra : op(carry, ra);
         cf |= rf;
@@ -371,25 +319,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             tf |= f;
         }
         T P = Op::neutral();
-        if constexpr (SPLIT) {
-            using A = typename Op::OpA;
-            using Sa = TileStatus<double>;
-            if (lane == 0) {
-                const NolbRecords<double> rec = split_records<Op>(status, ntiles);
-                rec.agg[tile] = tv.b;
-                rec.flag[tile] = tf;
-            }
-            if (tile == 0) {
-                if (lane == 0) Sa::publish(status, 0, kPrefix, tv.a);
-            } else {
-                const bool done = tf;
-                if (lane == 0) Sa::publish(status, tile, done ? kPrefix : kAggregate, tv.a);
-                const int64_t t0 = tile * S::TILE;
-                const bool first_head = (sb.heads[t0 >> 5] >> (t0 & 31)) & 1u;
-                if (!first_head) P.a = lookback(status, tile, A(), err);
-                if (!done && lane == 0) Sa::publish(status, tile, kPrefix, A()(P.a, tv.a));
-            }
-        } else if constexpr (NOLB) {
+        if constexpr (NOLB) {
             if (lane == 0) {
                 const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
                 rec.agg[tile] = tv;
@@ -405,7 +335,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             bool first_head = false;
             if constexpr (SEG) {
                 const int64_t t0 = tile * S::TILE;
-                first_head = (sb.heads[t0 >> 5] >> (t0 & 31)) & 1u;
+                first_head = (head_word(t0 >> 5) >> (t0 & 31)) & 1u;
             }
             if (!first_head) P = lookback(status, tile, op, err);
             if (!done && lane == 0) St::publish(status, tile, kPrefix, op(P, tv));
@@ -433,30 +363,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
             if constexpr (NOLB) pv[k] = v;
             y[k] = op(cid, v);
         }
-        if constexpr (DUAL) {
-            double ya[V];
-#pragma unroll
-            for (int k = 0; k < V; k++) ya[k] = y[k].a;
-            store_lane<double, V>(in.scan, base, n, aligned, ya);
-#pragma unroll
-            for (int k = 0; k < V; k++) {
-                const int64_t e = base + k;
-                if ((eb[r] >> k) & 1u) {
-                    const uint32_t wd = sb.heads[e >> 5];
-                    const uint32_t mask = (uint32_t)(((uint64_t)2 << (e & 31)) - 1);
-                    const int rk = sb.wcnt[e >> 5] + __popc(wd & mask);
-                    const int slot = sb.nz[rk - 1];
-                    if (SPLIT && ((needs >> (r * V + k)) & 1ull) && !s_winf[w]) {
-                        // the reduction of the segment that entered the tile
-                        // lacks the tile's incoming prefix: recorded
-                        s_fix_slot = slot;
-                        if constexpr (NOLB) s_fix_val = pv[k];
-                    } else {
-                        in.red[slot] = y[k].b;
-                    }
-                }
-            }
-        } else if constexpr (!REDUCE) {
+        if constexpr (!REDUCE) {
             store_lane<T, V>(out, base, n, aligned, y);
         } else {
 #pragma unroll
@@ -479,15 +386,9 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
     if constexpr (NOLB) {
         __syncthreads();
         if (threadIdx.x == 0) {
-            if constexpr (SPLIT) {
-                const NolbRecords<double> rec = split_records<Op>(status, ntiles);
-                rec.slot[tile] = s_fix_slot;
-                rec.val[tile] = s_fix_val.b;
-            } else {
-                const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
-                rec.slot[tile] = s_fix_slot;
-                rec.val[tile] = s_fix_val;
-            }
+            const NolbRecords<T> rec = NolbRecords<T>::at(status, ntiles);
+            rec.slot[tile] = s_fix_slot;
+            rec.val[tile] = s_fix_val;
         }
     }
 }
@@ -531,72 +432,51 @@ __device__ __forceinline__ void seg_reduce_fix_body(NolbRecords<typename Op::T> 
     __shared__ T s_v[W];
     __shared__ int s_f[W];
     const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-    const int64_t per = (g + kFixThreads - 1) / kFixThreads;
-    const int64_t j0 = (int64_t)tid * per;
-    const int64_t j1 = j0 + per < g ? j0 + per : g;
-    // Records are read kChunk at a time, all loads in flight together: one
-    // workgroup is latency-bound, not bandwidth-bound.
-    constexpr int kChunk = 8;
-    T v = Op::neutral();
-    int f = 0;
-    for (int64_t c0 = j0; c0 < j1; c0 += kChunk) {
-        T a[kChunk];
-        int af[kChunk];
-#pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            a[i] = c0 + i < j1 ? rec.agg[c0 + i] : Op::neutral();
-            af[i] = c0 + i < j1 ? rec.flag[c0 + i] : 0;
-        }
-#pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            if (c0 + i < j1) {
-                v = af[i] ? a[i] : op(v, a[i]);
-                f |= af[i];
-            }
-        }
-    }
-    // exclusive scan of the threads' (flag, value) pairs, in thread order
-    T iv = v;
-    int iff = f;
-    seg_wave_scan(iff, iv, op);
-    if (lane == kWave - 1) { s_v[w] = iv; s_f[w] = iff; }
-    __syncthreads();
-    if (tid == 0) {
-        T pv = Op::neutral();
-        int pf = 0;
-        for (int i = 0; i < W; i++) {
-            const T a = s_v[i];
-            const int af = s_f[i];
-            s_v[i] = pv;
-            s_f[i] = pf;
-            pv = af ? a : op(pv, a);
-            pf |= af;
-        }
-    }
-    __syncthreads();
-    T ev = shfl_up1(iv);
-    int ef = shfl_up1(iff);
-    if (lane == 0) { ev = Op::neutral(); ef = 0; }
-    T p = ef ? ev : op(s_v[w], ev);
     const T cid = Op::cvl_id();
-    for (int64_t c0 = j0; c0 < j1; c0 += kChunk) {
-        T a[kChunk], pv[kChunk];
-        int af[kChunk], sl[kChunk];
-#pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            const bool in = c0 + i < j1;
-            a[i] = in ? rec.agg[c0 + i] : Op::neutral();
-            af[i] = in ? rec.flag[c0 + i] : 0;
-            sl[i] = in ? rec.slot[c0 + i] : -1;
-            pv[i] = in ? rec.val[c0 + i] : Op::neutral();
+    // Rounds of kFixThreads consecutive records, one per thread (coalesced
+    // loads, the next round's issued before this round's scan).  Each round
+    // is a segmented scan across the workgroup in record order, entered
+    // with the fold of all earlier rounds (carry, the same in every thread).
+    auto load = [&](int64_t j, T &a, int &f, int &sl, T &pv) {
+        const bool in = j < g;
+        a = in ? rec.agg[j] : Op::neutral();
+        f = in ? rec.flag[j] : 0;
+        sl = in ? rec.slot[j] : -1;
+        pv = in ? rec.val[j] : Op::neutral();
+    };
+    T carry = Op::neutral();
+    int cf = 0;
+    T a, pv;
+    int f, sl;
+    load(tid, a, f, sl, pv);
+    for (int64_t base = 0; base < g; base += kFixThreads) {
+        T na, npv;
+        int nf, nsl;
+        load(base + kFixThreads + tid, na, nf, nsl, npv);
+        T iv = a;
+        int iff = f;
+        seg_wave_scan(iff, iv, op);
+        if (lane == kWave - 1) { s_v[w] = iv; s_f[w] = iff; }
+        __syncthreads();
+        T wp = carry;                               // everything before this wave's first record
+        int wf = cf;
+        for (int i = 0; i < w; i++) {
+            wp = s_f[i] ? s_v[i] : op(wp, s_v[i]);
+            wf |= s_f[i];
         }
-#pragma unroll
-        for (int i = 0; i < kChunk; i++) {
-            if (c0 + i < j1) {
-                if (sl[i] >= 0) out[sl[i]] = op(cid, op(p, pv[i]));
-                p = af[i] ? a[i] : op(p, a[i]);
-            }
+        T ev = shfl_up1(iv);
+        int ef = shfl_up1(iff);
+        if (lane == 0) { ev = Op::neutral(); ef = 0; }
+        const T p = ef ? ev : op(wp, ev);           // everything before this record
+        if (sl >= 0) out[sl] = op(cid, op(p, pv));
+        for (int i = w; i < W; i++) {               // and through the round's last record
+            wp = s_f[i] ? s_v[i] : op(wp, s_v[i]);
+            wf |= s_f[i];
         }
+        carry = wp;
+        cf = wf;
+        __syncthreads();                            // s_v is written again next round
+        a = na; f = nf; sl = nsl; pv = npv;
     }
 }
 

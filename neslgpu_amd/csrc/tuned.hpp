@@ -43,7 +43,7 @@
 #define CVLHIP_SEG_REDUCE_BLOCK 512
 #endif
 #ifndef CVLHIP_PAIR_ROWS
-#define CVLHIP_PAIR_ROWS 4
+#define CVLHIP_PAIR_ROWS 8
 #endif
 #ifndef CVLHIP_PAIR_BLOCK
 #define CVLHIP_PAIR_BLOCK 512

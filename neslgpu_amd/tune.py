@@ -56,7 +56,7 @@ SPACE = {
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
     "CVLHIP_SEG_REDUCE_BLOCK": ([256, 512, 1024], "max_red"),
     # a comma-joined key searches its macros together (values are tuples)
-    "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(r, b) for r in (4, 8) for b in (256, 512, 1024)], "add_sed_max_red"),
+    "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(4, 256), (8, 256), (4, 512), (8, 512), (4, 1024)], "add_sed_max_red"),
 }
 
 
