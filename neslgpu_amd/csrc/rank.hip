@@ -329,13 +329,19 @@ loaded:
         const KeyT k = key[j];
         const unsigned d = in ? (unsigned)((k >> a.shift) & 255) : 0u;
         dig[j] = d;
-        // lanes of this round holding the same digit (inactive lanes excluded)
-        uint64_t peers = ballot(in);
+        // lanes of this round holding the same digit (inactive lanes
+        // excluded): the lanes that differ from this one in some bit of the
+        // digit are the OR over the bits of (lanes with the bit set) XOR
+        // (all lanes if this lane has it set, none if not)
+        uint32_t dlo = 0, dhi = 0;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const uint64_t bb = ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bb : ~bb;
+            const uint32_t spread = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);   // 0 or ~0: bit b
+            const uint64_t bb = ballot(spread != 0u);
+            dlo |= (uint32_t)bb ^ spread;
+            dhi |= (uint32_t)(bb >> 32) ^ spread;
         }
+        const uint64_t peers = ~(((uint64_t)dhi << 32) | dlo) & ballot(in);
         const unsigned before = (unsigned)__popcll(peers & lt);
         const unsigned old = s_cnt[w][d];
         loc[j] = old + before;
