@@ -66,7 +66,7 @@ constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
 constexpr int kPartV = 4;
 static_assert(kPartItems % kPartV == 0, "items per thread come in runs of kPartV");
 constexpr int kMaxFan = 256;                   // children per region per pass
-static_assert(!kPartLookback || kPartThreads >= kMaxFan, "the look-back gives every child a thread");
+static_assert(kPartThreads >= kMaxFan, "every child has a thread for its slot");
 static_assert(kMaxFan == kTileScanCols, "first-pass histogram rows are scanned 256 wide");
 static_assert(kBlkLog4 <= 16 && kBlkLog8 <= 16, "offsets within a final block fit the 16 bits the last pass keeps");
 
@@ -309,8 +309,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
         }
     }
     __syncthreads();
-    // Exclusive scan of the 256 counts (one wave) and one global claim per
-    // non-empty bucket.
+    // Exclusive scan of the 256 counts (one wave), and the tile's slot in
+    // every child: from the column scan (first pass) or one global claim per
+    // non-empty bucket, one thread per bucket so the claims are in flight
+    // together.
     if (tid < kWave) {
         unsigned c[kMaxFan / kWave], tot = 0;
 #pragma unroll
@@ -319,11 +321,17 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(IndexSource src, Pai
         unsigned run = inc - tot;
 #pragma unroll
         for (int i = 0; i < kMaxFan / kWave; i++) {
-            const int d = tid * (kMaxFan / kWave) + i;
-            s_start[d] = run;
+            s_start[tid * (kMaxFan / kWave) + i] = run;
             run += c[i];
-            if (tile_ofs) s_gofs[d] = tile_ofs[bt * kMaxFan + d];
-            else if (!LB) s_gofs[d] = c[i] ? atomicAdd(cnt + first_child + d, c[i]) : 0u;
+        }
+    }
+    if (!LB && tid < kMaxFan) {
+        const int d = tid;
+        if (tile_ofs) {
+            s_gofs[d] = tile_ofs[bt * kMaxFan + d];
+        } else {
+            const unsigned c = s_hist[d];
+            s_gofs[d] = c ? atomicAdd(cnt + first_child + d, c) : 0u;
         }
     }
     if (LB && tid < kMaxFan) {

@@ -412,7 +412,12 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
 // give the same bits).  256 / 512 / 1024 threads: max_red of 2^26 doubles
 // 0.206 / 0.207 / 0.218 ms, the fused SpMV product (gathers in the loader)
 // 1.35 / 1.28 / 1.32 ms, fused quickhull 29.6 / 27.1 / 28.2 ms.
-constexpr int kSegReduceRows = 8;
+#ifndef CVLHIP_SEG_REDUCE_ROWS
+#define CVLHIP_SEG_REDUCE_ROWS 8
+#endif
+constexpr int kSegReduceRows = CVLHIP_SEG_REDUCE_ROWS;
+// Rows per wave of the largest tiles of segmented scans (the flat8 arm).
+constexpr int kSegScanRows = 8;
 #ifndef CVLHIP_SEG_REDUCE_BLOCK
 #define CVLHIP_SEG_REDUCE_BLOCK 512
 #endif

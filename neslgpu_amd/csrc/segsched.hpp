@@ -245,7 +245,10 @@ __global__ __launch_bounds__((sched_block<Op, REDUCE>())) void k_seg_sched(In in
         else if (arm == kArmWave) sched_wave<Op, REDUCE, In>(in, out, sv);
         else sched_block_arm<Op, REDUCE, In, B>(in, out, sv);
         return;
-    default: sched_flat<Op, kSegReduceRows, REDUCE, In, B, false>(in, out, n, sb, scratch, aligned, err); return;
+    default:
+        sched_flat<Op, REDUCE ? kSegReduceRows : kSegScanRows, REDUCE, In, B, false>(in, out, n, sb, scratch, aligned,
+                                                                                   err);
+        return;
     }
 }
 
@@ -272,7 +275,8 @@ inline int sched_grid(int64_t n) {
                                                            sched_block<Op, REDUCE>(), 0);
         resident = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
     }
-    constexpr int64_t tile1 = sched_tile<Op, REDUCE, 1>(), tile8 = sched_tile<Op, REDUCE, kSegReduceRows>();
+    constexpr int64_t tile1 = sched_tile<Op, REDUCE, 1>(),
+                      tile8 = sched_tile<Op, REDUCE, REDUCE ? kSegReduceRows : kSegScanRows>();
     const int64_t t1 = (n + tile1 - 1) / tile1, t8 = (n + tile8 - 1) / tile8;
     int64_t g = t1 < resident ? t1 : resident;
     if (g < t8) g = t8;

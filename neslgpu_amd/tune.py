@@ -55,6 +55,7 @@ SPACE = {
     "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
     "CVLHIP_SEG_REDUCE_BLOCK": ([256, 512, 1024], "max_red"),
+    "CVLHIP_SEG_REDUCE_ROWS": ([8, 12, 16], "max_red"),
     # a comma-joined key searches its macros together (values are tuples)
     "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(4, 256), (8, 256), (4, 512), (8, 512), (4, 1024)], "add_sed_max_red"),
 }
@@ -271,13 +272,20 @@ def measure(primitive: str, reps: int = 7) -> dict:
         r = torch.empty(m, dtype=torch.float64, device=dev)
         if primitive == "max_red":
             L.max_red(r, dx, sd, n, m, cv.scratch(L.max_red_scratch(n, m)))
+        elif primitive == "add_sed":
+            # a float sum: the scan's order of additions is not the serial one
+            sc = torch.empty_like(dx)
+            L.add_sed(sc, dx, sd, n, m, cv.scratch(L.add_sed_scratch(n, m)))
+            want = O.segscan("add", "d", x, lens)
+            ok = np.allclose(sc.cpu().numpy(), want, rtol=0, atol=1e-9 * (1 + np.abs(want).max()))
         else:
             sc = torch.empty_like(dx)
             L.cvlhip_add_sed_max_red(sc, r, dx, sd, n, m, cv.scratch(L.cvlhip_add_sed_max_red_scratch(n, m)))
-        ok = np.array_equal(r.cpu().numpy().view(np.int64), O.segreduce("max", "d", x, lens).view(np.int64))
+        if primitive != "add_sed":
+            ok = np.array_equal(r.cpu().numpy().view(np.int64), O.segreduce("max", "d", x, lens).view(np.int64))
     ok = bool(ok) and L.cvlhip_check_device_errors() == 0
     # timing at the benchmark's size
-    if primitive in ("max_red", "add_sed_max_red"):
+    if primitive in ("max_red", "add_sed", "add_sed_max_red"):
         n = 1 << 26
         x = torch.empty(n, dtype=torch.float64, device=dev)
         L.cvlhip_fill_splitmix_f64(x, n, 0x4E45534C00000003, -1.0, 1.0)
@@ -287,6 +295,10 @@ def measure(primitive: str, reps: int = 7) -> dict:
         if primitive == "max_red":
             scr = cv.scratch(L.max_red_scratch(n, m))
             fn = lambda: L.max_red(r, x, sd, n, m, scr)
+        elif primitive == "add_sed":
+            sc = torch.empty_like(x)
+            scr = cv.scratch(L.add_sed_scratch(n, m))
+            fn = lambda: L.add_sed(sc, x, sd, n, m, scr)
         else:
             sc = torch.empty_like(x)
             scr = cv.scratch(L.cvlhip_add_sed_max_red_scratch(n, m))
