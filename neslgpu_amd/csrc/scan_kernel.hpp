@@ -122,17 +122,19 @@ struct NolbRecords {
     int *flag;
     int *slot;
     T *val;
-    __host__ __device__ static int64_t up8(int64_t b) { return (b + 7) & ~(int64_t)7; }
-    __host__ __device__ static int64_t bytes(int64_t g) {
-        return up8(up8((int64_t)sizeof(T) * g) + 8 * g) + (int64_t)sizeof(T) * g;
-    }
+    // each array starts a multiple of 16 bytes after `base`
+    __host__ __device__ static int64_t up16(int64_t b) { return (b + 15) & ~(int64_t)15; }
+    __host__ __device__ static int64_t flag_at(int64_t g) { return up16((int64_t)sizeof(T) * g); }
+    __host__ __device__ static int64_t slot_at(int64_t g) { return up16(flag_at(g) + 4 * g); }
+    __host__ __device__ static int64_t val_at(int64_t g) { return up16(slot_at(g) + 4 * g); }
+    __host__ __device__ static int64_t bytes(int64_t g) { return val_at(g) + (int64_t)sizeof(T) * g; }
     __host__ __device__ static NolbRecords at(void *base, int64_t g) {
         char *b = (char *)base;
         NolbRecords r;
         r.agg = (T *)b;
-        r.flag = (int *)(b + up8((int64_t)sizeof(T) * g));
-        r.slot = r.flag + g;
-        r.val = (T *)(b + up8(up8((int64_t)sizeof(T) * g) + 8 * g));
+        r.flag = (int *)(b + flag_at(g));
+        r.slot = (int *)(b + slot_at(g));
+        r.val = (T *)(b + val_at(g));
         return r;
     }
 };
@@ -428,60 +430,105 @@ constexpr int kSegReduceBlock = CVLHIP_SEG_REDUCE_BLOCK;
 // in tile order, completes the segment that entered the tile:
 // out[slot[t]] = cvl_id op (prefix(t) op val[t]).
 constexpr int kFixThreads = 1024;
+constexpr int kFixPer = 8;   // consecutive records per thread and round
+
+// Records j0 .. j0+kFixPer-1 of one array (`fill` past g): 16-byte loads
+// when all of them exist and the array is 16-byte aligned (j0 is a
+// multiple of kFixPer).
+template <class T>
+__device__ __forceinline__ void fix_load(const T *__restrict__ p, int64_t j0, int64_t g, bool vec, T fill,
+                                         T (&x)[kFixPer]) {
+    typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+    typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "four- or eight-byte records");
+    if (vec && j0 + kFixPer <= g) {
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int q = 0; q < kFixPer / 4; q++) {
+                const u32x4_t w = *(const u32x4_t *)(p + j0 + 4 * q);
+                const unsigned e[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int c = 0; c < 4; c++) x[4 * q + c] = __builtin_bit_cast(T, e[c]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kFixPer / 2; q++) {
+                const u64x2_t w = *(const u64x2_t *)(p + j0 + 2 * q);
+                const unsigned long long e[2] = {w.x, w.y};
+#pragma unroll
+                for (int c = 0; c < 2; c++) x[2 * q + c] = __builtin_bit_cast(T, e[c]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFixPer; k++) x[k] = j0 + k < g ? p[j0 + k] : fill;
+    }
+}
+
 template <class Op>
 __device__ __forceinline__ void seg_reduce_fix_body(NolbRecords<typename Op::T> rec, int64_t g,
                                                     typename Op::T *__restrict__ out) {
     using T = typename Op::T;
     constexpr int W = kFixThreads / kWave;
+    constexpr int64_t ROUND = (int64_t)kFixThreads * kFixPer;
     const Op op;
     __shared__ T s_v[W];
     __shared__ int s_f[W];
     const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
     const T cid = Op::cvl_id();
-    // Rounds of kFixThreads consecutive records, one per thread (coalesced
-    // loads, the next round's issued before this round's scan).  Each round
-    // is a segmented scan across the workgroup in record order, entered
-    // with the fold of all earlier rounds (carry, the same in every thread).
-    auto load = [&](int64_t j, T &a, int &f, int &sl, T &pv) {
-        const bool in = j < g;
-        a = in ? rec.agg[j] : Op::neutral();
-        f = in ? rec.flag[j] : 0;
-        sl = in ? rec.slot[j] : -1;
-        pv = in ? rec.val[j] : Op::neutral();
-    };
+    const bool vec = (((uintptr_t)rec.agg | (uintptr_t)rec.flag | (uintptr_t)rec.slot | (uintptr_t)rec.val) & 15) == 0;
+    // Rounds of ROUND records (one round up to 8192 tiles).  Each thread
+    // folds its kFixPer consecutive records in registers; one segmented scan
+    // across the workgroup, entered with the fold of all earlier rounds
+    // (carry, the same in every thread), gives each thread what precedes its
+    // first record; the thread then walks its records again and completes
+    // the segments that entered them.  That is a few instructions per
+    // record, against a wave scan (hundreds of instructions) per 64 records
+    // when each thread holds one: the one workgroup is bound by issue.
+    // (16 records per thread were slower: each 16-byte load of a wave then
+    // spans 8 KiB, twice the cache lines.)
     T carry = Op::neutral();
     int cf = 0;
-    T a, pv;
-    int f, sl;
-    load(tid, a, f, sl, pv);
-    for (int64_t base = 0; base < g; base += kFixThreads) {
-        T na, npv;
-        int nf, nsl;
-        load(base + kFixThreads + tid, na, nf, nsl, npv);
-        T iv = a;
-        int iff = f;
+    for (int64_t base = 0; base < g; base += ROUND) {
+        const int64_t j0 = base + (int64_t)tid * kFixPer;
+        T a[kFixPer], pv[kFixPer];
+        int f[kFixPer], sl[kFixPer];
+        fix_load(rec.agg, j0, g, vec, Op::neutral(), a);
+        fix_load(rec.flag, j0, g, vec, 0, f);
+        fix_load(rec.slot, j0, g, vec, -1, sl);
+        fix_load(rec.val, j0, g, vec, Op::neutral(), pv);
+        T iv = Op::neutral();
+        int iff = 0;
+#pragma unroll
+        for (int k = 0; k < kFixPer; k++) {
+            iv = f[k] ? a[k] : op(iv, a[k]);
+            iff |= f[k];
+        }
         seg_wave_scan(iff, iv, op);
         if (lane == kWave - 1) { s_v[w] = iv; s_f[w] = iff; }
         __syncthreads();
-        T wp = carry;                               // everything before this wave's first record
-        int wf = cf;
-        for (int i = 0; i < w; i++) {
-            wp = s_f[i] ? s_v[i] : op(wp, s_v[i]);
-            wf |= s_f[i];
+        // the fold through each wave: wp stops before this wave's first
+        // thread, tp runs through the round's last
+        T tp = carry, wp = carry;
+        int tf = cf;
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            if (i == w) wp = tp;
+            tp = s_f[i] ? s_v[i] : op(tp, s_v[i]);
+            tf |= s_f[i];
         }
         T ev = shfl_up1(iv);
         int ef = shfl_up1(iff);
         if (lane == 0) { ev = Op::neutral(); ef = 0; }
-        const T p = ef ? ev : op(wp, ev);           // everything before this record
-        if (sl >= 0) out[sl] = op(cid, op(p, pv));
-        for (int i = w; i < W; i++) {               // and through the round's last record
-            wp = s_f[i] ? s_v[i] : op(wp, s_v[i]);
-            wf |= s_f[i];
+        T p = ef ? ev : op(wp, ev);                 // everything before this thread's first record
+#pragma unroll
+        for (int k = 0; k < kFixPer; k++) {
+            if (sl[k] >= 0) out[sl[k]] = op(cid, op(p, pv[k]));
+            p = f[k] ? a[k] : op(p, a[k]);
         }
-        carry = wp;
-        cf = wf;
+        carry = tp;
+        cf = tf;
         __syncthreads();                            // s_v is written again next round
-        a = na; f = nf; sl = nsl; pv = npv;
     }
 }
 

@@ -20,7 +20,8 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-# elements per tile of a segmented reduction: 512 threads x rows x 16 bytes
+# elements in 512 threads x rows x 16 bytes: a multiple of the tile of a
+# segmented reduction in workgroups of 512 threads or fewer
 def tile(ty, rows):
     return 8 * rows * 64 * (4 if ty == "z" else 2)
 
@@ -83,4 +84,40 @@ def test_segmented_reductions_at_tile_boundaries(cv, ty, rows):
                     assert np.all(np.abs(got - want) <= fp_add_tol_vec(absum, lens)), (name, op, how)
                 else:
                     eq(got, want, f"{name} {op}_re{ty} {how}")
+    assert L.cvlhip_check_device_errors() == 0
+
+
+def test_second_step_over_many_rounds(cv):
+    """More tiles than the second step folds in one round (8192 records):
+    2^27 + 40000 int32 in 8-row tiles of up to 256 threads are more than
+    16384 tiles (and the 2- and 1-row tiles many times that), with
+    segments of geometric(2^16) lengths crossing tile and round boundaries,
+    a few empty ones among them."""
+    import torch
+    rng = np.random.default_rng(81)
+    n = (1 << 27) + 40000
+    lens = rng.geometric(1.0 / (1 << 16), n >> 15).astype(np.int64)
+    lens[rng.integers(0, lens.size, 40)] = 0
+    c = np.cumsum(lens)
+    lens = lens[: np.searchsorted(c, n)]
+    lens = np.append(lens, n - lens.sum()).astype(np.int32)
+    m = lens.size
+    x = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
+    sd = cv.segd(lens)
+    dx = torch.from_numpy(x).to(cv.device)
+    out = torch.empty(m, dtype=dx.dtype, device=cv.device)
+    L = cv.lib
+    from neslgpu_amd import dispatch as D
+    for op in ("max", "add"):
+        want = O.segreduce(op, "z", x, lens)
+        runs = [("entry", lambda: getattr(L, f"{op}_rez")(
+            out, dx, sd, n, m, cv.scratch(getattr(L, f"{op}_rez_scratch")(n, m))))]
+        for arm in ("flat8", "flat2", "flat1"):     # the tiled schedules
+            runs.append((arm, lambda arm=arm: L.cvlhip_seg_reduce(
+                D.OPS[op], 0, D.ARMS.index(arm), out, dx, sd, n, m, cv.scratch(L.cvlhip_seg_scratch(n, m)))))
+        for how, run in runs:
+            print("many rounds", op, how, n, m, flush=True)
+            out.fill_(0x5A5A5A5A)
+            run()
+            eq(host(out), want, f"{op}_rez {how} over many rounds")
     assert L.cvlhip_check_device_errors() == 0
