@@ -3,7 +3,8 @@
 // of each of the 256 digits (tile-major: a histogram kernel writes its row as
 // one contiguous 1 KiB store, and the reordering kernel reads it back the
 // same way).  After the scan, entry [t][d] is the number of elements with
-// digit d in tiles before t, and totals[d] (when non-null) the column total.
+// digit d in tiles before t, totals[d] (when non-null) the column total and
+// starts[d] (when non-null) the sum of the totals of the columns before d.
 //
 // One kernel: workgroup g owns rows [g*R, (g+1)*R) and thread d column d.
 // It sums its R counts, publishes the sum in a status word (flag | value,
@@ -25,7 +26,8 @@ constexpr int kTileScanCols = 256;
 __global__ __launch_bounds__(kTileScanCols) inline void k_tile_scan(unsigned *__restrict__ m, int64_t tiles,
                                                                     unsigned *__restrict__ totals,
                                                                     uint64_t *__restrict__ status,
-                                                                    unsigned *__restrict__ err) {
+                                                                    unsigned *__restrict__ err,
+                                                                    unsigned *__restrict__ starts) {
     constexpr int R = kTileScanRows;
     const int d = threadIdx.x;
     const int64_t g = blockIdx.x;
@@ -69,7 +71,20 @@ __global__ __launch_bounds__(kTileScanCols) inline void k_tile_scan(unsigned *__
         if (r0 + r < tiles) m[(r0 + r) * kTileScanCols + d] = run;
         run += v[r];
     }
-    if (totals && g == (int64_t)gridDim.x - 1) totals[d] = run;
+    if (g == (int64_t)gridDim.x - 1) {
+        if (totals) totals[d] = run;
+        if (starts) {
+            // the columns' starts: exclusive scan of the totals across them
+            __shared__ unsigned s_w[kTileScanCols / kWave];
+            const int w = d / kWave, lane = d % kWave;
+            const unsigned inc = wave_inclusive_scan(run, AddU());
+            if (lane == kWave - 1) s_w[w] = inc;
+            __syncthreads();
+            unsigned base = 0;
+            for (int i = 0; i < w; i++) base += s_w[i];
+            starts[d] = base + inc - run;
+        }
+    }
 }
 
 // Status words a scan of `tiles` rows needs (64-bit words).
@@ -79,12 +94,15 @@ inline int64_t tile_scan_status_words(int64_t tiles) {
 
 // Scan m (tiles x 256) in place on the library stream; `status` holds
 // tile_scan_status_words(tiles) words and is cleared here unless the caller
-// cleared it already (`cleared`).
-inline void tile_scan(unsigned *m, int64_t tiles, unsigned *totals, uint64_t *status, bool cleared = false) {
+// cleared it already (`cleared`).  `starts` (optional) receives the
+// exclusive scan of the column totals: where each column's run begins.
+inline void tile_scan(unsigned *m, int64_t tiles, unsigned *totals, uint64_t *status, bool cleared = false,
+                      unsigned *starts = nullptr) {
     if (tiles <= 0) return;
     const int64_t groups = (tiles + kTileScanRows - 1) / kTileScanRows;
     if (!cleared) fill_u32(status, 0, 2 * kTileScanCols * groups);
-    k_tile_scan<<<(unsigned)groups, kTileScanCols, 0, g_stream>>>(m, tiles, totals, status, device_error_word());
+    k_tile_scan<<<(unsigned)groups, kTileScanCols, 0, g_stream>>>(m, tiles, totals, status, device_error_word(),
+                                                                  starts);
     LAUNCHCHECK();
 }
 

@@ -505,119 +505,146 @@ int64_t partition_permute_scratch_bytes(int64_t n, int width) {
     return 2 * pair + counters + hist + 256;
 }
 
+// The scratch of one partition run: the two pair buffers (the last pass
+// writes Local16 into the space of the buffer it would otherwise fill:
+// 2 + sizeof(T) <= the pair's bytes per element), one child-counter array
+// per level, then the first pass's column-scan status and histogram.
+// clear() zeroes the counters and the status (one launch).
 template <class T>
-static void run_partition(void *d, const void *s, IndexSource src, int64_t n, void *scratch) {
-    unsigned *err = device_error_word();
-    constexpr int kBlkLog = blk_log<T>();
-    // Region sizes per pass: from ceil-pow2(n) down to the final block, at
-    // most 8 bits per pass.
-    int nlog = 0;
-    while (((int64_t)1 << nlog) < n) nlog++;
-    if (nlog <= kBlkLog) {
-        k_block_scatter<T, true><<<1, kPartThreads, 0, g_stream>>>(src, Pairs<T>{}, (const T *)s, (T *)d, n, nullptr,
-                                                                   err);
-        LAUNCHCHECK();
-        return;
-    }
-    if (!scratch) usage_error("permute called without scratch");
-    auto up = up256;
-    char *b = (char *)scratch;
+struct PartLayout {
+    int nlog = 0, levels = 0;
     Pairs<T> buf[2];
-    // The last pass writes Local16 into the space of the pair buffer it
-    // would otherwise fill (2 + sizeof(T) <= the pair's bytes per element).
     Local16<T> local[2];
-    for (int k = 0; k < 2; k++) {
-        char *const b0 = b;
-        if constexpr (sizeof(T) == 4) {
-            buf[k].p = (uint2 *)b;
-            b += up(8 * n);
-        } else {
-            buf[k].idx = (int *)b;
-            b += up(4 * n);
-            buf[k].val = (T *)b;
-            b += up((int64_t)sizeof(T) * n);
-        }
-        local[k].idx = (unsigned short *)b0;
-        local[k].val = (T *)(b0 + up(2 * n));
+    unsigned *cnts[4] = {};
+    uint64_t *status0 = nullptr;
+    unsigned *hist0 = nullptr;
+    int64_t grid0 = 0;            // tiles of the first pass (one region)
+    // Region sizes per pass: from ceil-pow2(n) down to the final block, at
+    // most 8 bits per pass; full 8-bit passes first, the odd bits last (the
+    // first pass takes its slots from the histogram, later passes claim
+    // them with one atomic per (tile, child), so the narrower fan belongs
+    // there).
+    int bits(int l, int plog) const { return l + 1 < levels ? 8 : plog - blk_log<T>(); }
+    char *clear_from = nullptr, *clear_to = nullptr;
+    void clear() const {
+        if (clear_to > clear_from) fill_u32((unsigned *)clear_from, 0, (int64_t)(clear_to - clear_from) / 4);
     }
-    const int levels = (nlog - kBlkLog + 7) / 8;
-    // Per level: its child counters (>= 256 entries: the column scan writes
-    // 256 totals); then the first pass's column-scan status and histogram.
-    // The counters and the status are contiguous and cleared in one launch.
-    unsigned *cnts[4];
-    {
+    PartLayout(int64_t n, void *scratch) {
+        while (((int64_t)1 << nlog) < n) nlog++;
+        if (nlog <= blk_log<T>()) return;   // one block: no scratch
+        if (!scratch) usage_error("permute called without scratch");
+        auto up = up256;
+        char *b = (char *)scratch;
+        for (int k = 0; k < 2; k++) {
+            char *const b0 = b;
+            if constexpr (sizeof(T) == 4) {
+                buf[k].p = (uint2 *)b;
+                b += up(8 * n);
+            } else {
+                buf[k].idx = (int *)b;
+                b += up(4 * n);
+                buf[k].val = (T *)b;
+                b += up((int64_t)sizeof(T) * n);
+            }
+            local[k].idx = (unsigned short *)b0;
+            local[k].val = (T *)(b0 + up(2 * n));
+        }
+        levels = (nlog - blk_log<T>() + 7) / 8;
+        // per level: its child counters (>= 256 entries: the column scan
+        // writes 256 totals)
         int pl = nlog;
         for (int l = 0; l < levels; l++) {
-            const int bits = l + 1 < levels ? 8 : pl - kBlkLog;
-            const int cl = pl - bits;
+            const int cl = pl - bits(l, pl);
             const int64_t nchild = ((n - 1) >> cl) + 1;
             cnts[l] = (unsigned *)b;
             b += up(4 * ((nchild > kMaxFan ? nchild : kMaxFan) + 1));
             pl = cl;
         }
+        grid0 = (n + kPartTile - 1) / kPartTile;
+        // the column scan's status, or the look-back's tile counter; then the
+        // histogram, or the look-back words (cleared with the counters)
+        status0 = (uint64_t *)b;
+        b += up(8 * tile_scan_status_words(grid0));
+        hist0 = (unsigned *)b;
+        if (kPartLookback) b += up(4 * (int64_t)kMaxFan * grid0);
+        clear_from = (char *)cnts[0];
+        clear_to = b;
     }
-    const int64_t grid0 = (n + kPartTile - 1) / kPartTile;    // tiles of the first pass (one region)
-    // the column scan's status, or the look-back's tile counter; then the
-    // histogram, or the look-back words (cleared with the counters)
-    uint64_t *status0 = (uint64_t *)b;
-    b += up(8 * tile_scan_status_words(grid0));
-    unsigned *hist0 = (unsigned *)b;
-    if (kPartLookback) b += up(4 * (int64_t)kMaxFan * grid0);
-    fill_u32(cnts[0], 0, (int64_t)((char *)b - (char *)cnts[0]) / 4);
-    if (!kPartLookback) b += up(4 * (int64_t)kMaxFan * grid0);
-    int plog = nlog;
-    int cur = -1;
-    unsigned *cnt = nullptr, *prev = nullptr;
-    for (int l = 0; l < levels; l++) {
-        // Full 8-bit passes first, the odd bits last: the first pass takes its
-        // slots from the histogram, later passes claim them with one atomic
-        // per (tile, child), so the narrower fan belongs there.
-        const int bits = l + 1 < levels ? 8 : plog - kBlkLog;
-        const int clog = plog - bits;
-        prev = cnt;
-        cnt = cnts[l];
+};
+
+// Levels l0 .. levels-1 after the first (their input: buffer `cur`, its
+// children counted in cnts[l0 - 1]), then the block scatter into d.
+template <class T>
+static void partition_later_levels(const PartLayout<T> &L, IndexSource src, void *d, int64_t n, int l0, int cur,
+                                   unsigned *err) {
+    int plog = L.nlog;
+    for (int l = 0; l < l0; l++) plog -= L.bits(l, plog);
+    unsigned *cnt = L.cnts[l0 - 1];
+    for (int l = l0; l < L.levels; l++) {
+        const int clog = plog - L.bits(l, plog);
+        unsigned *const prev = cnt;
+        cnt = L.cnts[l];
         const int64_t regions = ((n - 1) >> plog) + 1;
         const int64_t rsize = (int64_t)1 << plog;
         const int64_t tpr = (rsize < n ? rsize + kPartTile - 1 : n + kPartTile - 1) / kPartTile;
         const int64_t grid = regions * tpr;
         const int out = cur == 0 ? 1 : 0;
-        const bool last = l + 1 == levels;        // children are the final blocks: write Local16
-        if (cur < 0) {
-            // first pass: one region, many tiles -> slots from a histogram + column scan
-            if (grid != grid0) usage_error("permute: first-pass tiling");
-            unsigned *hist = kPartLookback ? nullptr : hist0;
-            uint32_t *lb = kPartLookback ? hist0 : nullptr;
-            unsigned *lb_ctr = kPartLookback ? (unsigned *)status0 : nullptr;
-            if (!kPartLookback) {
-                uint64_t *status = status0;
-                k_part_hist<<<(unsigned)grid, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
-                LAUNCHCHECK();
-                tile_scan(hist, grid, cnt, status, true);
-            }
-            if (last)
-                k_partition<T, true, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, Pairs<T>{}, (const T *)s, local[out], n, plog, clog, tpr, nullptr, cnt, hist, err, lb,
-                    lb_ctr);
-            else
-                k_partition<T, true><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, Pairs<T>{}, (const T *)s, buf[out], n, plog, clog, tpr, nullptr, cnt, hist, err, lb,
-                    lb_ctr);
-        } else {
-            if (last)
-                k_partition<T, false, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, buf[cur], nullptr, local[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
-            else
-                k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
-                    src, buf[cur], nullptr, buf[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
-        }
+        if (l + 1 == L.levels)        // children are the final blocks: write Local16
+            k_partition<T, false, Local16<T>><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                src, L.buf[cur], nullptr, L.local[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
+        else
+            k_partition<T, false><<<(unsigned)grid, kPartThreads, 0, g_stream>>>(
+                src, L.buf[cur], nullptr, L.buf[out], n, plog, clog, tpr, prev, cnt, nullptr, err);
         LAUNCHCHECK();
         cur = out;
         plog = clog;
     }
-    const int64_t blocks = ((n - 1) >> kBlkLog) + 1;
+    const int64_t blocks = ((n - 1) >> blk_log<T>()) + 1;
+    if (L.levels == l0) usage_error("permute: no level after the first");
     k_block_scatter<T, false, Local16<T>><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(
-        src, local[cur], nullptr, (T *)d, n, cnt, err);
+        src, L.local[cur], nullptr, (T *)d, n, cnt, err);
     LAUNCHCHECK();
+}
+
+template <class T>
+static void run_partition(void *d, const void *s, IndexSource src, int64_t n, void *scratch) {
+    unsigned *err = device_error_word();
+    const PartLayout<T> L(n, scratch);
+    L.clear();
+    if (L.levels == 0) {
+        k_block_scatter<T, true><<<1, kPartThreads, 0, g_stream>>>(src, Pairs<T>{}, (const T *)s, (T *)d, n, nullptr,
+                                                                   err);
+        LAUNCHCHECK();
+        return;
+    }
+    // first pass: one region, many tiles -> slots from a histogram + column
+    // scan (or the look-back)
+    const int plog = L.nlog;
+    const int clog = plog - L.bits(0, plog);
+    const int64_t tpr = L.grid0;
+    unsigned *cnt = L.cnts[0];
+    unsigned *hist = kPartLookback ? nullptr : L.hist0;
+    uint32_t *lb = kPartLookback ? L.hist0 : nullptr;
+    unsigned *lb_ctr = kPartLookback ? (unsigned *)L.status0 : nullptr;
+    if (!kPartLookback) {
+        k_part_hist<<<(unsigned)L.grid0, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
+        LAUNCHCHECK();
+        tile_scan(hist, L.grid0, cnt, L.status0, true);
+    }
+    if (L.levels == 1) {
+        k_partition<T, true, Local16<T>><<<(unsigned)L.grid0, kPartThreads, 0, g_stream>>>(
+            src, Pairs<T>{}, (const T *)s, L.local[0], n, plog, clog, tpr, nullptr, cnt, hist, err, lb, lb_ctr);
+        LAUNCHCHECK();
+        const int64_t blocks = ((n - 1) >> blk_log<T>()) + 1;
+        k_block_scatter<T, false, Local16<T>><<<(unsigned)blocks, kPartThreads, 0, g_stream>>>(
+            src, L.local[0], nullptr, (T *)d, n, cnt, err);
+        LAUNCHCHECK();
+        return;
+    }
+    k_partition<T, true><<<(unsigned)L.grid0, kPartThreads, 0, g_stream>>>(
+        src, Pairs<T>{}, (const T *)s, L.buf[0], n, plog, clog, tpr, nullptr, cnt, hist, err, lb, lb_ctr);
+    LAUNCHCHECK();
+    partition_later_levels<T>(L, src, d, n, 1, 0, err);
 }
 
 template <class T>
@@ -627,6 +654,29 @@ void partition_permute(void *d, const void *s, const int *p, const void *sd, int
     const bool vec = !segmented && ((uintptr_t)p & 15) == 0 && ((uintptr_t)s & 15) == 0;
     IndexSource src{p, SegdView(sd, n, sd ? m : 0), segmented, 0, SegdView(), vec};
     run_partition<T>(d, s, src, n, scratch);
+}
+
+// The inverse of a sort order in two steps: the caller runs the first
+// partition pass (the rank's last pass, which knows each element's final
+// position as it places it) and partition_invert_finish the rest.
+InvertFirstPass partition_invert_begin(int64_t n, void *scratch) {
+    InvertFirstPass f{};
+    if (n <= 0) return f;
+    const PartLayout<int> L(n, scratch);
+    if (L.levels < 2) return f;          // one partition pass or none: partition_invert
+    L.clear();
+    f.pairs = L.buf[0].p;
+    f.cnt = L.cnts[0];
+    f.clog = L.nlog - L.bits(0, L.nlog);
+    return f;
+}
+
+void partition_invert_finish(int *rank, int64_t n, void *scratch) {
+    const PartLayout<int> L(n, scratch);   // the layout of partition_invert_begin, counters kept
+    if (L.levels < 2) usage_error("partition_invert_finish: no split invert for %lld elements", (long long)n);
+    // later passes read only the pair buffers; the source is not consulted
+    const IndexSource src{nullptr, SegdView(), false, 1, SegdView(), true};
+    partition_later_levels<int>(L, src, rank, n, 1, 0, device_error_word());
 }
 
 void partition_invert(int *rank, const int *order, int64_t n, const void *sd, int64_t m, void *scratch) {

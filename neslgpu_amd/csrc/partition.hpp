@@ -14,4 +14,21 @@ int64_t partition_permute_scratch_bytes(int64_t n, int width);
 // elements, m > 1 segments) is given: the inverse of a sort order.
 void partition_invert(int *rank, const int *order, int64_t n, const void *sd, int64_t m, void *scratch);
 
+// The same inverse (unsegmented) in two steps, for a caller that places
+// the sorted elements itself: partition_invert_begin lays out `scratch`
+// (partition_permute_scratch_bytes(n, 4)) and clears its counters; the
+// caller then writes, for every position k, the pair (order[k], k) into
+// child order[k] >> clog of `pairs` -- child c's pairs at [c << clog,
+// (c << clog) + cnt[c]), cnt[] the children's totals, which the caller
+// also writes; partition_invert_finish runs the remaining passes.  With
+// pairs == nullptr (n of at most 2^21 elements: one partition pass or
+// none) the caller uses partition_invert instead.
+struct InvertFirstPass {
+    uint2 *pairs;
+    unsigned *cnt;
+    int clog;
+};
+InvertFirstPass partition_invert_begin(int64_t n, void *scratch);
+void partition_invert_finish(int *rank, int64_t n, void *scratch);
+
 }  // namespace cvlhip

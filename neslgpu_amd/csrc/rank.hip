@@ -107,6 +107,100 @@ __global__ __launch_bounds__(kBlock) void k_key_hist(const void *__restrict__ s,
         if ((&sh[0][0])[i]) atomicAdd(hist + i, (&sh[0][0])[i]);
 }
 
+// Which passes run.  A pass whose digit takes one value over all keys keeps
+// the order and is skipped; the digit takes one value iff none of its bits
+// is 1 in one key and 0 in another, i.e. shares no bit with (OR of the keys)
+// AND (OR of the complemented keys).  The keys' OR words come from the
+// first pass's histogram kernel; those of the segment numbers (segmented
+// ranks: the numbers of the non-empty segments) from k_seg_or, as `words`
+// [0] (OR of the numbers) and [1] (OR of their complements), one per
+// 256-byte line, in kOrCopies copies (workgroup b adds to copy
+// b % kOrCopies, so fewer atomics queue on each word).
+constexpr int kOrStride = 64;
+constexpr int kOrWords = 2;
+constexpr int kOrCopies = 8;
+struct OrU {
+    using T = uint32_t;
+    __host__ __device__ static constexpr uint32_t neutral() { return 0u; }
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a | b; }
+};
+
+// The workgroup's OR of v[0..NW) into words[first..first+NW): one atomic
+// per word and workgroup.
+template <int NW>
+__device__ __forceinline__ void or_publish(const uint32_t (&v)[NW], unsigned *__restrict__ words, int first) {
+    __shared__ uint32_t s_or[kBlock / kWave][NW];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        const uint32_t r = wave_reduce(v[k], OrU());
+        if (lane == 0) s_or[w][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < NW) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < kBlock / kWave; i++) r |= s_or[i][threadIdx.x];
+        if (r) atomicOr(words + ((blockIdx.x % kOrCopies) * kOrWords + first + threadIdx.x) * kOrStride, r);
+    }
+}
+
+// The runs: the per-tile words of the first pass's keys (k_rank_hist with
+// ORW: bits that differ from key 0, two words per tile) folded with the
+// segment words of k_seg_or (when segmented); bit p of the posted mask is
+// set when pass p reorders something (key passes first, then segment
+// passes).  One workgroup.
+constexpr int kOrPostThreads = 1024;
+__global__ __launch_bounds__(kOrPostThreads) void k_or_post(const uint32_t *__restrict__ parts, int64_t nparts,
+                                                           const uint32_t *__restrict__ seg_words, int key_passes,
+                                                           int seg_passes, Mailbox *mb, unsigned long long seq) {
+    __shared__ uint32_t s_or[kOrPostThreads / kWave][2];
+    const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+    uint32_t v[2] = {0, 0};
+    for (int64_t t = tid; t < nparts; t += kOrPostThreads) {
+        const uint2 q = *(const uint2 *)(parts + 2 * t);
+        v[0] |= q.x;
+        v[1] |= q.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const uint32_t r = wave_reduce(v[k], OrU());
+        if (lane == 0) s_or[w][k] = r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t o[2] = {0, 0};
+        for (int i = 0; i < kOrPostThreads / kWave; i++)
+            for (int k = 0; k < 2; k++) o[k] |= s_or[i][k];
+        const uint64_t kdiff = ((uint64_t)o[1] << 32) | o[0];
+        uint32_t so = 0, sc = 0;
+        if (seg_words)
+            for (int c = 0; c < kOrCopies; c++) {
+                so |= seg_words[(c * kOrWords + 0) * kOrStride];
+                sc |= seg_words[(c * kOrWords + 1) * kOrStride];
+            }
+        const uint32_t sdiff = so & sc;
+        unsigned long long mask = 0;
+        for (int p = 0; p < key_passes; p++)
+            if ((kdiff >> (8 * p)) & 255u) mask |= 1ull << p;
+        for (int q = 0; q < seg_passes; q++)
+            if ((sdiff >> (8 * q)) & 255u) mask |= 1ull << (key_passes + q);
+        mailbox_post(mb, seq, mask);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_or(const int *__restrict__ lens, int64_t m,
+                                                   unsigned *__restrict__ words) {
+    uint32_t o = 0, c = 0;
+    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < m; g += (int64_t)gridDim.x * kBlock)
+        if (lens[g] != 0) {
+            o |= (uint32_t)g;
+            c |= ~(uint32_t)g;
+        }
+    const uint32_t v[2] = {o, c};
+    or_publish<2>(v, words, 0);
+}
+
 // Segment-number digits: segment g contributes lens[g] elements.
 __global__ void k_seg_hist(const int *__restrict__ lens, int64_t m, int npass, unsigned *__restrict__ hist) {
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < m; g += (int64_t)gridDim.x * blockDim.x) {
@@ -156,6 +250,14 @@ struct PassArgs {
     int up;
     SegdView seg;                  // segment of an element (segmented ranks)
     unsigned *err;                 // device error word
+    // The last pass of an unsegmented rank, fused with the first partition
+    // pass of the inverse (partition_invert_begin): instead of the order it
+    // writes the pairs (index, position) into child index >> clog of
+    // `pairs`, at the tile's slot from tile_ofs2 (tiles x 256, the column
+    // scan of the per-tile child counts that k_rank_hist wrote to hist2).
+    uint2 *pairs;
+    const unsigned *tile_ofs2;
+    int clog;
 };
 
 // Key of element e for this pass (ix = its original index).
@@ -198,11 +300,26 @@ __device__ __forceinline__ void store_key(const PassArgs &a, int64_t pos, KeyT k
 
 // Per-tile digit counts of one pass, as a tiles x 256 matrix (tile t =
 // elements [t*kRankTile, (t+1)*kRankTile) of the current order).
-template <class KeyT>
-__global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned *__restrict__ hist) {
+// With B2 (the fused last pass) it also counts each tile's elements per
+// child of the inverse's first partition pass (index >> a.clog) into hist2.
+// With ORW (the first pass, launched before the host knows which passes
+// run) it also writes the bits in which the tile's keys differ from key 0
+// to or_parts[2 tile], [2 tile + 1] (low, high 32 bits) for k_or_post.
+template <class KeyT, bool B2, bool ORW = false>
+__global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned *__restrict__ hist,
+                                                            unsigned *__restrict__ hist2,
+                                                            uint32_t *__restrict__ or_parts = nullptr) {
     __shared__ unsigned s_hist[kRadix];
+    __shared__ unsigned s_hist2[B2 ? kRadix : 1];
     const int tid = threadIdx.x;
-    for (int i = tid; i < kRadix; i += kRankThreads) s_hist[i] = 0;
+    // ORW: key 0 of the vector, the reference every tile compares with
+    // (read first, so that its latency hides behind the tile's loads)
+    KeyT k0 = 0;
+    if constexpr (ORW) k0 = pass_key<KeyT>(a, 0, 0);
+    for (int i = tid; i < kRadix; i += kRankThreads) {
+        s_hist[i] = 0;
+        if constexpr (B2) s_hist2[i] = 0;
+    }
     __syncthreads();
     const int64_t tile = blockIdx.x;      // rows are whole 1 KiB stores: launch order
     const int64_t t0 = tile * kRankTile;
@@ -236,25 +353,82 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_hist(PassArgs a, unsigned
 #pragma unroll
     for (int j = 0; j < kRankItems; j++)
         if (in[j]) atomicAdd(&s_hist[(kv[j] >> a.shift) & 255], 1u);
+    if constexpr (ORW) {
+        // the bits in which some key of the tile differs from key 0 of the
+        // vector (the same reference in every tile): a digit takes one value
+        // over all keys iff none of its bits differs anywhere
+        constexpr int NW = sizeof(KeyT) / 4;
+        __shared__ uint32_t s_or[kRankThreads / kWave][NW];
+        KeyT x = 0;
+#pragma unroll
+        for (int j = 0; j < kRankItems; j++)
+            if (in[j]) x |= kv[j] ^ k0;
+        const int w = tid / kWave, lane = tid % kWave;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const uint32_t r = wave_reduce((uint32_t)((uint64_t)x >> (32 * k)), OrU());
+            if (lane == 0) s_or[w][k] = r;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            uint32_t r = 0;
+            if (tid < NW)
+#pragma unroll
+                for (int i = 0; i < kRankThreads / kWave; i++) r |= s_or[i][tid];
+            or_parts[2 * tile + tid] = r;
+        }
+    }
+    if constexpr (B2) {
+        // the elements' original indices: children of the inverse
+#pragma unroll
+        for (int g = 0; g < kRankItems / 4; g++) {
+            const int64_t e0 = t0 + ((int64_t)g * kRankThreads + tid) * 4;
+            int ix[4];
+            if (a.idx_in && e0 + 4 <= a.n) {
+                const u32x4 w = *(const u32x4 *)((const uint32_t *)a.idx_in + e0);
+                ix[0] = (int)w.x; ix[1] = (int)w.y; ix[2] = (int)w.z; ix[3] = (int)w.w;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) ix[c] = e0 + c < a.n ? (a.idx_in ? a.idx_in[e0 + c] : (int)(e0 + c)) : -1;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (e0 + c < a.n && (unsigned)ix[c] < (unsigned)a.n) atomicAdd(&s_hist2[(unsigned)ix[c] >> a.clog], 1u);
+        }
+    }
     __syncthreads();
-    for (int d = tid; d < kRadix; d += kRankThreads) hist[tile * kRadix + d] = s_hist[d];
+    for (int d = tid; d < kRadix; d += kRankThreads) {
+        hist[tile * kRadix + d] = s_hist[d];
+        if constexpr (B2) hist2[tile * kRadix + d] = s_hist2[d];
+    }
 }
 
 // KOUT: bytes per key written to keys_out (a.kout), 0 when no keys are.
-template <class KeyT, int KOUT>
+// FUSE (KOUT 0): the pass writes the inverse's first partition pass
+// (a.pairs) instead of the order.
+template <class KeyT, int KOUT, bool FUSE = false>
 __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
+    static_assert(!FUSE || KOUT == 0, "the fused last pass writes no keys");
     constexpr int W = kRankThreads / kWave;
+    // keys, then (reinterpreted) indices, in digit order; or (FUSE) the
+    // 8-byte pairs in child order
+    constexpr int kStage = FUSE ? kRankTile * 8 / (int)sizeof(KeyT) : kRankTile;
     __shared__ unsigned s_cnt[W][kRadix];     // per-wave digit counts, then per-wave starts within a digit
     __shared__ unsigned s_tot[kRadix];
     __shared__ unsigned s_lstart[kRadix];     // tile-local start of each digit's run
     __shared__ unsigned s_gbase[kRadix];      // global position of the run
-    __shared__ KeyT s_stage[kRankTile];       // keys, then (reinterpreted) indices, in digit order
-    __shared__ unsigned char s_dig[kRankTile];
+    __shared__ KeyT s_stage[kStage];
+    __shared__ unsigned char s_dig[FUSE ? 1 : kRankTile];
+    __shared__ unsigned s_c2[FUSE ? kRadix : 1];   // FUSE: per-child counts, then starts, of the tile
+    __shared__ unsigned s_g2[FUSE ? kRadix : 1];   // FUSE: the tile's slot in each child
+    __shared__ unsigned s_placed;                  // FUSE: pairs staged
     __shared__ unsigned s_tile;
 
     const int tid = threadIdx.x;
     const int w = tid / kWave, lane = tid % kWave;
     for (int i = tid; i < W * kRadix; i += kRankThreads) (&s_cnt[0][0])[i] = 0;
+    if constexpr (FUSE)
+        for (int i = tid; i < kRadix; i += kRankThreads) s_c2[i] = 0;
     // a chained pass numbers its tiles in the order they start; the
     // order of the tiles' runs within a digit is then their data order
     if (kRankChain && a.chain && tid == 0) s_tile = atomicAdd(a.ticket, 1u);
@@ -385,6 +559,62 @@ loaded:
     }
     __syncthreads();
 
+    if constexpr (FUSE) {
+        // Every element's sorted position is known here: place the pair
+        // (index, position) in child index >> clog of the inverse's first
+        // partition pass, staged in child order, each child's run written
+        // contiguously at the tile's slot in that child.
+        uint2 *s_pair = (uint2 *)s_stage;
+        unsigned ch[kRankItems], rk[kRankItems], pos[kRankItems];
+#pragma unroll
+        for (int j = 0; j < kRankItems; j++) {
+            const int64_t e = wb + (int64_t)j * kWave + lane;
+            ch[j] = 0xFFFFFFFFu;
+            if (e < a.n) {
+                const unsigned d = dig[j];
+                pos[j] = s_gbase[d] + s_lstart[d] + s_cnt[w][d] + loc[j];
+                if ((unsigned)idx[j] < (unsigned)a.n) {
+                    ch[j] = (unsigned)idx[j] >> a.clog;
+                    rk[j] = atomicAdd(&s_c2[ch[j]], 1u);
+                } else {
+                    atomicOr(a.err, 1u);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < kWave) {          // child starts within the tile (one wave)
+            unsigned c[kRadix / kWave], t = 0;
+#pragma unroll
+            for (int i = 0; i < kRadix / kWave; i++) { c[i] = s_c2[tid * (kRadix / kWave) + i]; t += c[i]; }
+            unsigned run = wave_inclusive_scan(t, AddU()) - t;
+#pragma unroll
+            for (int i = 0; i < kRadix / kWave; i++) {
+                s_c2[tid * (kRadix / kWave) + i] = run;
+                run += c[i];
+            }
+            if (tid == kWave - 1) s_placed = run;
+        }
+        for (int i = tid; i < kRadix; i += kRankThreads) s_g2[i] = a.tile_ofs2[tile * kRadix + i];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kRankItems; j++)
+            if (ch[j] != 0xFFFFFFFFu) s_pair[s_c2[ch[j]] + rk[j]] = make_uint2((unsigned)idx[j], pos[j]);
+        __syncthreads();
+        // s_pair[0 .. placed): the tile's elements with a valid index, by
+        // child (index < n < 2^(clog + 8), so every child is below 256)
+        const int placed = (int)s_placed;
+        const int64_t csize = (int64_t)1 << a.clog;
+        for (int i = tid; i < placed; i += kRankThreads) {
+            const uint2 pr = s_pair[i];
+            const unsigned c = pr.x >> a.clog;
+            const int64_t at = (int64_t)c * csize + s_g2[c] + (i - s_c2[c]);
+            const int64_t cend = ((int64_t)c + 1) * csize < a.n ? ((int64_t)c + 1) * csize : a.n;
+            if (at < cend) a.pairs[at] = pr;
+            else atomicOr(a.err, 1u);
+        }
+        return;
+    }
+
     // Stage in digit order, then write each digit's run contiguously:
     // first the keys (when a later pass needs them), then the indices.
     unsigned slot[kRankItems];
@@ -444,7 +674,7 @@ __global__ void k_rank_identity(int *rank, int64_t n, SegdView seg, int segmente
 
 // ---------------------------------------------------------------- host driver
 struct RankScratch {
-    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, scan_status, invert, bytes;
+    int64_t keysA, keysB, idxA, idxB, hist, starts, counters, status, scan_status, status2, orparts, invert, bytes;
     RankScratch(int64_t n, bool dbl) {
         auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
         int64_t kw = dbl ? 8 : 4;
@@ -458,8 +688,14 @@ struct RankScratch {
         counters = up(starts + 4 * 12 * kRadix);
         status = up(counters + 4 * 16);
         scan_status = up(status + 4 * kRadix * (tiles > 0 ? tiles : 1));
-        // one column-scan status region per pass (at most 12), cleared together
-        invert = up(scan_status + 12 * 8 * tile_scan_status_words(tiles > 0 ? tiles : 1));
+        // one column-scan status region per pass (at most 12), one for the
+        // fused last pass's child counts and one for the speculative first
+        // pass, cleared together
+        status2 = up(scan_status + 14 * 8 * tile_scan_status_words(tiles > 0 ? tiles : 1));
+        // the fused last pass's per-tile child counts; the first pass's
+        // per-tile OR words
+        orparts = up(status2 + 4 * kRadix * (tiles > 0 ? tiles : 1));
+        invert = up(orparts + 8 * (tiles > 0 ? tiles : 1));
         bytes = up(invert + partition_permute_scratch_bytes(n, 4));
     }
 };
@@ -487,34 +723,87 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     const int total = key_passes + seg_passes;
     const int64_t tiles = (n + kRankTile - 1) / kRankTile;
 
-    fill_u32(hist, 0, 12 * kRadix);
-    {
-        int g = grid_for(n, kBlock * 16);
-        if (g > 2048) g = 2048;
-        const bool aligned = ((uintptr_t)s & 15) == 0;
+    const bool chain = kRankChain && n < (int64_t)kChainMask;
+    const int64_t sw = tile_scan_status_words(tiles);
+    if (!g_hist_host) HIPCHECK(hipHostMalloc(&g_hist_host, 4 * 12 * kRadix, hipHostMallocDefault));
+    const bool aligned = ((uintptr_t)s & 15) == 0;
+    int g = grid_for(n, kBlock * 16);
+    if (g > 2048) g = 2048;
+    int gs = grid_for(m, kBlock);
+    if (gs > 4096) gs = 4096;
+    bool runs[12] = {};
+    // The speculative first histogram (below): pass 0's tile counts, column
+    // scanned with its own status region (index 13).
+    unsigned *tile_ofs2 = (unsigned *)(b + L.status2);
+    InvertFirstPass inv{};
+    bool spec = false;
+    if (chain) {
+        // chained passes need every digit's start up front: full histograms
+        fill_u32(hist, 0, 12 * kRadix);
         if (dbl) k_key_hist<true><<<g, kBlock, 0, g_stream>>>(s, n, up, aligned, hist);
         else k_key_hist<false><<<g, kBlock, 0, g_stream>>>(s, n, up, aligned, hist);
         LAUNCHCHECK();
         if (seg_passes) {
-            int gs = grid_for(m, kBlock);
-            if (gs > 4096) gs = 4096;
             k_seg_hist<<<gs, kBlock, 0, g_stream>>>(seg.lens, m, seg_passes, hist + key_passes * kRadix);
             LAUNCHCHECK();
         }
         k_hist_scan<<<total, kRadix, 0, g_stream>>>(hist, starts);
         LAUNCHCHECK();
+        HIPCHECK(hipMemcpyAsync(g_hist_host, hist, 4 * (size_t)total * kRadix, hipMemcpyDeviceToHost, g_stream));
+        HIPCHECK(hipStreamSynchronize(g_stream));
+        for (int p = 0; p < total; p++) {
+            int nonzero = 0;
+            for (int d = 0; d < kRadix; d++) nonzero += g_hist_host[p * kRadix + d] != 0;
+            runs[p] = nonzero > 1;
+        }
+    } else {
+        // Pass 0's tile counts, which nearly always serve, also give the
+        // keys' OR words; k_or_post posts which passes run to the host.
+        // Behind it the device clears the scan status regions and the
+        // inverse's counters and column-scans pass 0's counts, covering the
+        // host's wait and its next launches.
+        unsigned *words = hist;
+        static_assert(kOrCopies * kOrWords * kOrStride <= 12 * kRadix, "the OR words fit the histogram space");
+        if (seg_passes) {
+            fill_u32(words, 0, kOrCopies * kOrWords * kOrStride);
+            k_seg_or<<<gs, kBlock, 0, g_stream>>>(seg.lens, m, words);
+            LAUNCHCHECK();
+        }
+        PassArgs a0{};
+        a0.n = n;
+        a0.up = up;
+        a0.seg = seg;
+        a0.source = dbl ? 1 : 0;
+        a0.src = s;
+        a0.shift = 0;
+        a0.vec4 = !dbl && aligned;
+        a0.err = device_error_word();
+        uint32_t *or_parts = (uint32_t *)(b + L.orparts);
+        if (dbl)
+            k_rank_hist<uint64_t, false, true><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a0, tile_ofs, nullptr,
+                                                                                              or_parts);
+        else
+            k_rank_hist<uint32_t, false, true><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a0, tile_ofs, nullptr,
+                                                                                              or_parts);
+        LAUNCHCHECK();
+        const unsigned long long seq = mailbox_next();
+        k_or_post<<<1, kOrPostThreads, 0, g_stream>>>(or_parts, tiles, seg_passes ? words : nullptr, key_passes,
+                                                      seg_passes, mailbox_device(), seq);
+        LAUNCHCHECK();
+        fill_u32(scan_status, 0, 2 * sw * 14);      // status regions: 12 passes, the inverse's counts, pass 0's
+        tile_scan(tile_ofs, tiles, nullptr, scan_status + 13 * sw, true, starts);
+        spec = true;
+        // An unsegmented rank whose inverse takes two or more partition
+        // passes runs the first of them inside its last pass (FUSE): the pass
+        // knows every element's sorted position as it places it, so the
+        // order is never written and read back.
+        if (!segmented) inv = partition_invert_begin(n, b + L.invert);
+        const unsigned long long mask = mailbox_wait(seq);
+        for (int p = 0; p < total; p++) runs[p] = (mask >> p) & 1u;
     }
-    // Which passes reorder anything?  (A pass whose 256 counts hold one
-    // non-zero bucket keeps the order and is skipped.)
-    if (!g_hist_host) HIPCHECK(hipHostMalloc(&g_hist_host, 4 * 12 * kRadix, hipHostMallocDefault));
-    HIPCHECK(hipMemcpyAsync(g_hist_host, hist, 4 * (size_t)total * kRadix, hipMemcpyDeviceToHost, g_stream));
-    HIPCHECK(hipStreamSynchronize(g_stream));
     int run[12], nrun = 0;
-    for (int p = 0; p < total; p++) {
-        int nonzero = 0;
-        for (int d = 0; d < kRadix; d++) nonzero += g_hist_host[p * kRadix + d] != 0;
-        if (nonzero > 1) run[nrun++] = p;
-    }
+    for (int p = 0; p < total; p++)
+        if (runs[p]) run[nrun++] = p;
     if (nrun == 0) {
         k_rank_identity<<<grid_for(n, kBlock * 4), kBlock, 0, g_stream>>>(rank, n, seg, segmented);
         LAUNCHCHECK();
@@ -523,15 +812,15 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
     // The last key pass that runs decides whether keys must still travel.
     int last_key_run = -1;
     for (int i = 0; i < nrun; i++) if (run[i] < key_passes) last_key_run = i;
+    // pass 0's speculative counts serve when pass 0 runs, unless it is also
+    // the fused last pass, which counts the inverse's children as well
+    spec = spec && run[0] == 0 && !(inv.pairs && nrun == 1);
 
-    const int64_t sw = tile_scan_status_words(tiles);
     // Chained passes: per-digit counts below 2^30 (n < 2^30); one set of
     // words for all passes (alternating state sets) and a tile counter per
     // pass, contiguous and cleared together.
-    const bool chain = kRankChain && n < (int64_t)kChainMask;
     unsigned *tickets = (unsigned *)(b + L.counters);
     if (chain) fill_u32(tickets, 0, (L.status - L.counters) / 4 + kRadix * tiles);
-    else fill_u32(scan_status, 0, 2 * sw * nrun);       // the status regions of every pass that runs
     void *keys[2] = {b + L.keysA, b + L.keysB};
     int *idxs[2] = {(int *)(b + L.idxA), (int *)(b + L.idxB)};
     int cur = -1;   // buffer holding the current (keys, idx); -1 = original input
@@ -572,14 +861,29 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         }
         a.idx_out = idxs[out];
         a.err = device_error_word();
-        if (!chain) {
-            if (dbl) k_rank_hist<uint64_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
-            else k_rank_hist<uint32_t><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs);
+        const bool fuse = inv.pairs && i + 1 == nrun;
+        a.pairs = fuse ? inv.pairs : nullptr;
+        a.tile_ofs2 = tile_ofs2;
+        a.clog = inv.clog;
+        if (!chain && !(i == 0 && spec)) {
+            if (fuse) {
+                if (dbl) k_rank_hist<uint64_t, true><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs, tile_ofs2);
+                else k_rank_hist<uint32_t, true><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs, tile_ofs2);
+            } else {
+                if (dbl) k_rank_hist<uint64_t, false><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs, nullptr);
+                else k_rank_hist<uint32_t, false><<<(unsigned)tiles, kRankThreads, 0, g_stream>>>(a, tile_ofs, nullptr);
+            }
             LAUNCHCHECK();
-            tile_scan(tile_ofs, tiles, nullptr, scan_status + i * sw, true);
+            // the column scan also leaves the digits' starts (the scan of its totals)
+            tile_scan(tile_ofs, tiles, nullptr, scan_status + i * sw, true, starts + p * kRadix);
+            // the children's totals go to the inverse's counters
+            if (fuse) tile_scan(tile_ofs2, tiles, inv.cnt, scan_status + 12 * sw, true);
         }
         const dim3 grid((unsigned)tiles), block(kRankThreads);
-        if (dbl) {
+        if (fuse) {
+            if (dbl) k_rank_pass<uint64_t, 0, true><<<grid, block, 0, g_stream>>>(a);
+            else k_rank_pass<uint32_t, 0, true><<<grid, block, 0, g_stream>>>(a);
+        } else if (dbl) {
             if (a.keys_out) k_rank_pass<uint64_t, 8><<<grid, block, 0, g_stream>>>(a);
             else k_rank_pass<uint64_t, 0><<<grid, block, 0, g_stream>>>(a);
         } else if (!a.keys_out) {
@@ -599,7 +903,8 @@ void rank_run(int *rank, const void *s, int64_t n, bool dbl, bool up, const void
         cur = out;
     }
     // rank = inverse of the final order
-    partition_invert(rank, idxs[cur], n, segmented ? sd : nullptr, m, b + L.invert);
+    if (inv.pairs) partition_invert_finish(rank, n, b + L.invert);
+    else partition_invert(rank, idxs[cur], n, segmented ? sd : nullptr, m, b + L.invert);
 }
 
 int64_t rank_scratch_bytes(int64_t n, bool dbl) { return RankScratch(n, dbl).bytes; }

@@ -17,6 +17,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 N = 300_007            # many rank tiles and a ragged last one
+# above 2^21 keys the inverse of the sort order takes two partition passes,
+# and an unsegmented rank's last pass runs the first of them itself
+N2 = (1 << 21) + 4101
 
 
 @pytest.fixture(scope="module")
@@ -29,6 +32,8 @@ def keys(kind, n, rng):
     r = rng.integers(-2 ** 31, 2 ** 31, n, dtype=np.int64).astype(np.uint32)
     masks = {
         "all": 0xFFFFFFFF,     # digits 0-3 vary
+        "d3": 0xFF000000,      # digit 3 only: one pass, which is also the first
+        "d0": 0x000000FF,      # digit 0 only: one pass, pass 0
         "d13": 0xFF00FF00,     # digits 1 and 3 (0 and 2 constant): 32 -> 8
         "d12": 0x00FFFF00,     # digits 1 and 2: 32 -> 16, last pass keyless
         "d03": 0xFF0000FF,     # digits 0 and 3: 32 -> 8
@@ -39,7 +44,10 @@ def keys(kind, n, rng):
     return (r & np.uint32(masks[kind])).view(np.int32)
 
 
-KINDS = ["all", "d13", "d12", "d03", "d123", "small"]
+# pass 0 runs first ("all", "d03", "small", "d0") or is skipped (the rest),
+# which decides whether its tile counts, taken while the host reads which
+# digits vary, are used
+KINDS = ["all", "d13", "d12", "d03", "d123", "small", "d3", "d0"]
 
 
 def device(cv, a, shift):
@@ -82,4 +90,38 @@ def test_segmented_rank_digit_patterns(cv, kind):
         getattr(L, name)(d, device(cv, x, 0), sd, N, lens.size,
                          cv.scratch(getattr(L, name + "_scratch")(N, lens.size)))
         eq(host(d), O.rank(x, up, lens), f"{name} {kind}")
+    assert L.cvlhip_check_device_errors() == 0
+
+
+FUSED_KINDS = ["all", "d12", "d3", "small", "d0"]
+
+
+@pytest.mark.parametrize("kind", FUSED_KINDS)
+def test_rank_last_pass_fused_with_inverse(cv, kind):
+    """Above 2^21 keys: the last pass places (index, position) pairs for the
+    inverse's first partition pass instead of writing the order -- with
+    every pass running, with the last pass also the first (digit 3 or digit
+    0 only: indices are positions; for digit 0 the pass also counts the
+    inverse's children, so pass 0's early counts go unused), with a keyless
+    last pass, and for doubles with ties, up and down, aligned and not."""
+    import torch
+    rng = np.random.default_rng(FUSED_KINDS.index(kind) + 300)
+    L = cv.lib
+    x = keys(kind, N2, rng)
+    for up in (True, False):
+        for shift in (0, 1):
+            name = f"rk{'u' if up else 'd'}_luz"
+            print(name, kind, N2, "shift", shift, flush=True)
+            d = torch.empty(N2, dtype=torch.int32, device=cv.device)
+            getattr(L, name)(d, device(cv, x, shift), N2, cv.scratch(getattr(L, name + "_scratch")(N2)))
+            eq(host(d), O.rank(x, up), f"{name} {kind} n {N2} shift {shift}")
+    if kind == "all":
+        y = np.round(rng.uniform(-1e3, 1e3, N2), 1)          # many ties
+        dy = torch.from_numpy(y).to(cv.device)
+        for up in (True, False):
+            name = f"rk{'u' if up else 'd'}_lud"
+            print(name, N2, flush=True)
+            d = torch.empty(N2, dtype=torch.int32, device=cv.device)
+            getattr(L, name)(d, dy, N2, cv.scratch(getattr(L, name + "_scratch")(N2)))
+            eq(host(d), O.rank(y, up), f"{name} n {N2}")
     assert L.cvlhip_check_device_errors() == 0
