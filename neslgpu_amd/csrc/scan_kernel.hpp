@@ -448,7 +448,10 @@ __device__ __forceinline__ void fix_load(const T *__restrict__ p, int64_t j0, in
                 const u32x4_t w = *(const u32x4_t *)(p + j0 + 4 * q);
                 const unsigned e[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                for (int c = 0; c < 4; c++) x[4 * q + c] = __builtin_bit_cast(T, e[c]);
+                for (int c = 0; c < 4; c++) {
+                    const unsigned u = e[c];
+                    x[4 * q + c] = __builtin_bit_cast(T, u);
+                }
             }
         } else {
 #pragma unroll
@@ -456,7 +459,10 @@ __device__ __forceinline__ void fix_load(const T *__restrict__ p, int64_t j0, in
                 const u64x2_t w = *(const u64x2_t *)(p + j0 + 2 * q);
                 const unsigned long long e[2] = {w.x, w.y};
 #pragma unroll
-                for (int c = 0; c < 2; c++) x[2 * q + c] = __builtin_bit_cast(T, e[c]);
+                for (int c = 0; c < 2; c++) {
+                    const unsigned long long u = e[c];
+                    x[2 * q + c] = __builtin_bit_cast(T, u);
+                }
             }
         }
     } else {
