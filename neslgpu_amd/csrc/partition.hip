@@ -56,6 +56,13 @@ constexpr int blk_log() { return sizeof(T) == 4 ? kBlkLog4 : kBlkLog8; }
 #define CVLHIP_PART_LOOKBACK 0
 #endif
 constexpr bool kPartLookback = CVLHIP_PART_LOOKBACK != 0;
+// 1: the first pass claims its slots like the later passes, one atomic per
+// (tile, child), instead of a histogram pass over the indices and a
+// column scan.
+#ifndef CVLHIP_PART_ATOMIC_FIRST
+#define CVLHIP_PART_ATOMIC_FIRST 0
+#endif
+constexpr bool kPartAtomicFirst = CVLHIP_PART_ATOMIC_FIRST != 0 && !kPartLookback;
 constexpr int kPartThreads = CVLHIP_PART_THREADS;
 constexpr int kPartItems = CVLHIP_PART_ITEMS;
 constexpr int kPartTile = kPartThreads * kPartItems;   // pairs per tile
@@ -623,10 +630,10 @@ static void run_partition(void *d, const void *s, IndexSource src, int64_t n, vo
     const int clog = plog - L.bits(0, plog);
     const int64_t tpr = L.grid0;
     unsigned *cnt = L.cnts[0];
-    unsigned *hist = kPartLookback ? nullptr : L.hist0;
+    unsigned *hist = (kPartLookback || kPartAtomicFirst) ? nullptr : L.hist0;
     uint32_t *lb = kPartLookback ? L.hist0 : nullptr;
     unsigned *lb_ctr = kPartLookback ? (unsigned *)L.status0 : nullptr;
-    if (!kPartLookback) {
+    if (!kPartLookback && !kPartAtomicFirst) {
         k_part_hist<<<(unsigned)L.grid0, kPartThreads, 0, g_stream>>>(src, n, clog, hist);
         LAUNCHCHECK();
         tile_scan(hist, L.grid0, cnt, L.status0, true);

@@ -507,13 +507,18 @@ loaded:
         // excluded): the lanes that differ from this one in some bit of the
         // digit are the OR over the bits of (lanes with the bit set) XOR
         // (all lanes if this lane has it set, none if not)
+        // Per bit four instructions: the sign-extended bit, its lane mask
+        // (a compare of that value, which the compiler would otherwise
+        // rebuild from a shift of the digit), and one 3-input bit operation
+        // (v_bitop3, gfx950) per half of the mask: acc | (spread ^ mask),
+        // truth table 0xde over (spread, acc, mask).
         uint32_t dlo = 0, dhi = 0;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const uint32_t spread = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);   // 0 or ~0: bit b
-            const uint64_t bb = ballot(spread != 0u);
-            dlo |= (uint32_t)bb ^ spread;
-            dhi |= (uint32_t)(bb >> 32) ^ spread;
+            const uint64_t bb = __builtin_amdgcn_uicmp(spread, 0u, 33);            // lanes with bit b (ICMP_NE)
+            dlo = __builtin_amdgcn_bitop3_b32(spread, dlo, (uint32_t)bb, 0xde);
+            dhi = __builtin_amdgcn_bitop3_b32(spread, dhi, (uint32_t)(bb >> 32), 0xde);
         }
         const uint64_t peers = ~(((uint64_t)dhi << 32) | dlo) & ballot(in);
         const unsigned before = (unsigned)__popcll(peers & lt);

@@ -51,6 +51,7 @@ SPACE = {
     "CVLHIP_PART_ITEMS": ([4, 8, 12], "smp_puz"),
     "CVLHIP_PART_THREADS": ([512, 1024], "smp_puz"),
     "CVLHIP_BLK_LOG4": ([12, 13, 14], "smp_puz"),
+    "CVLHIP_PART_ATOMIC_FIRST": ([0, 1], "smp_puz"),
     "CVLHIP_RANK_THREADS": ([256, 512, 1024], "rku_luz"),
     "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
