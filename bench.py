@@ -243,21 +243,23 @@ def program_workloads():
     """(name, program text, stdin, fused library or None, graphs, repeats,
     description) of the whole-program workloads."""
     from neslgpu_amd import vinterp as V
-    from neslgpu_amd.build import FUSED_DIR, generated
-    qs = V.set_function_int_const(V.set_size(V.read(os.path.join(PROGRAMS, "qs.1000.vcode")), 1 << 20),
-                                  "QS_4", 100, 2147483647)
-    qh = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.vcode")), 1 << 24)
-    qhf = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.fcode")), 1 << 24)
+    from neslgpu_amd.build import generated
     from neslgpu_amd import workloads as W
+    qs = W.qs_bench_program(V.read(os.path.join(PROGRAMS, "qs.1000.vcode")))
+    qsf = V.read(generated("qs_bench", "fcode"))          # the same program through this package's fuser
+    qh = V.set_size(V.read(os.path.join(PROGRAMS, "quickhull.1000.vcode")), 1 << 24)
+    qhf = V.set_size(V.read(generated("quickhull.1000", "fcode")), 1 << 24)
     rows = W.spmv_rows()
     spmv_in = W.spmv_stdin(rows, W.SPMV_ROWS)
     spmv = V.read(os.path.join(OWN_PROGRAMS, "spmv.vcode"))
     spmvf = V.read(generated("spmv", "fcode"))
     return [
         ("quicksort", qs, None, None, False, 1, "qs.nesl: 2^20 keys uniform in [0, 2^31-1) (RAND), sorted"),
+        ("quicksort_fused", qsf, None, generated("qs_bench", "so"), False, 1,
+         "same, fused by neslgpu_amd.fusion.fuser (FUSED super-ops, generated HIP)"),
         ("quickhull", qh, None, None, False, 1, "quickhull.nesl: 2^24 points uniform in the unit disk"),
-        ("quickhull_fused", qhf, None, os.path.join(FUSED_DIR, "quickhull.1000.so"), False, 1,
-         "same, FUSED super-ops (generated HIP)"),
+        ("quickhull_fused", qhf, None, generated("quickhull.1000", "so"), False, 1,
+         "same, fused by neslgpu_amd.fusion.fuser (FUSED super-ops, generated HIP)"),
         ("spmv", spmv, spmv_in, None, False, 10, "y = A x, CSR 2^22 rows, power-law lengths (mean 16, "
          "max 2^16), 2^26 nnz, float64; BPERMUTE, *, +_REDUCE"),
         ("spmv_fused", spmvf, spmv_in, generated("spmv", "so"), False, 10,

@@ -91,6 +91,23 @@ def spmv_y_program(text: str) -> str:
     return t.replace(_SUM_TAIL, "RET", 1)
 
 
+QS_KEYS = 1 << 20
+
+
+def qs_bench_program(text: str) -> str:
+    """qs.1000.vcode as the benchmark runs it: 2^20 keys drawn uniform in
+    [0, 2^31 - 1) (the size constant of the entry function and QS_4's RAND
+    range, 100 in the shipped program)."""
+    from . import vinterp as V
+    return V.set_function_int_const(V.set_size(text, QS_KEYS), "QS_4", 100, 2147483647)
+
+
+def qs_bench_full_program(text: str) -> str:
+    """The same, printing the sorted keys (the interpreter's full result)."""
+    from . import vinterp as V
+    return V.full_result(qs_bench_program(text))
+
+
 def spmv_inputs(rows: np.ndarray, ncols: int, rnd_wuz):
     """(cols, vals, x) exactly as the SpMV program draws them with RAND:
     cols = RAND(ncols) per nonzero, then vals = RAND(2^30) / 2^29 - 1 per

@@ -33,3 +33,19 @@ def test_fuser_output_runs_on_the_gpu(name):
     want = open(os.path.join(GOLD, name + ".out")).read()
     assert stable(r.stdout, rec["volatile_stdout_lines"]) == stable(want, rec["volatile_stdout_lines"])
     assert r.stderr == open(os.path.join(GOLD, name + ".err")).read()
+
+
+def test_fused_quicksort_at_the_benchmark_size():
+    """qs.nesl at 2^20 keys in [0, 2^31 - 1), fused by this package's fuser
+    at build time (the benchmark's quicksort_fused, printing its keys): the
+    sorted keys must hash to what the serial CVL printed for the unfused
+    program (the qs.1048576.full digest)."""
+    import hashlib
+    rec = MANIFEST["qs.1048576.full"]
+    fcode, lib = generated("qs_bench_full", "fcode"), generated("qs_bench_full", "so")
+    assert os.path.exists(fcode) and os.path.exists(lib), f"{fcode} not built (python -m neslgpu_amd.build)"
+    print("fused quicksort 2^20", flush=True)
+    r = V.run(open(fcode).read(), fused=lib, timeout=240)
+    assert len(r.stdout.splitlines()) == rec["stdout_lines"]
+    got = stable(r.stdout, rec["volatile_stdout_lines"])
+    assert hashlib.sha256("\n".join(got).encode()).hexdigest() == rec["sha256"]
