@@ -30,6 +30,9 @@
 #ifndef CVLHIP_BLK_LOG4
 #define CVLHIP_BLK_LOG4 13
 #endif
+#ifndef CVLHIP_PART_ATOMIC_FIRST
+#define CVLHIP_PART_ATOMIC_FIRST 0
+#endif
 #ifndef CVLHIP_RANK_THREADS
 #define CVLHIP_RANK_THREADS 512
 #endif
@@ -39,11 +42,11 @@
 #ifndef CVLHIP_RANK_CHAIN
 #define CVLHIP_RANK_CHAIN 0
 #endif
-#ifndef CVLHIP_SEG_REDUCE_BLOCK
-#define CVLHIP_SEG_REDUCE_BLOCK 256
-#endif
 #ifndef CVLHIP_SEG_REDUCE_ROWS
 #define CVLHIP_SEG_REDUCE_ROWS 8
+#endif
+#ifndef CVLHIP_SEG_REDUCE_BLOCK
+#define CVLHIP_SEG_REDUCE_BLOCK 256
 #endif
 #ifndef CVLHIP_PAIR_ROWS
 #define CVLHIP_PAIR_ROWS 8
