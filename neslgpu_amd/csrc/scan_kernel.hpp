@@ -17,10 +17,13 @@ template <> struct Vec16<double> { typedef double type __attribute__((ext_vector
 // Elements each lane holds per row: one 16-byte load's worth, by default.
 template <class Op> struct lane_values { static constexpr int value = 16 / (int)sizeof(typename Op::T); };
 
-template <class Op, int R, int B = kBlock>
+// VB: bytes each lane holds per row; 16 (one 16-byte load) unless the
+// caller asks for wider lanes (the flat segmented-reduction tiles).
+template <class Op, int R, int B = kBlock, int VB = 16>
 struct ScanShape {
     using T = typename Op::T;
-    static constexpr int V = lane_values<Op>::value;   // elements per lane per row
+    static_assert(VB % 16 == 0, "whole 16-byte loads per lane");
+    static constexpr int V = VB == 16 ? lane_values<Op>::value : VB / (int)sizeof(T);   // elements per lane per row
     static constexpr int ROW = kWave * V;            // elements per wave row
     static constexpr int WAVES = B / kWave;
     static constexpr int WAVE_ELEMS = R * ROW;
@@ -36,15 +39,22 @@ template <> struct ordered<MinD> { static constexpr bool value = true; };
 #endif
 constexpr int kScanRows = CVLHIP_SCAN_ROWS;   // rows per wave in one tile
 
-// Load V elements starting at element `base` (neutral beyond n).
+// Load V elements starting at element `base` (neutral beyond n): whole
+// 16-byte loads when aligned and V is a multiple of a load's elements.
 template <class Op, int V>
 __device__ __forceinline__ void load_lane(const typename Op::T *__restrict__ in, int64_t base, int64_t n,
                                           bool aligned, typename Op::T (&x)[V]) {
     using T = typename Op::T;
-    if (aligned && base + V <= n) {
-        typename Vec16<T>::type v = *(const typename Vec16<T>::type *)(in + base);
+    constexpr int E = 16 / (int)sizeof(T);          // elements per 16-byte load
+    constexpr bool VEC = V % E == 0;                // (callers with fewer elements pass aligned = false)
+    if (VEC && aligned && base + V <= n) {
+        typename Vec16<T>::type v[VEC ? V / E : 1];
 #pragma unroll
-        for (int k = 0; k < V; k++) x[k] = load_val<Op>(v[k]);
+        for (int q = 0; q < V / E; q++) v[q] = *(const typename Vec16<T>::type *)(in + base + q * E);
+#pragma unroll
+        for (int q = 0; q < V / E; q++)
+#pragma unroll
+            for (int k = 0; k < E; k++) x[q * E + k] = load_val<Op>(v[q][k]);
     } else {
 #pragma unroll
         for (int k = 0; k < V; k++) x[k] = base + k < n ? load_val<Op>(in[base + k]) : Op::neutral();
@@ -150,15 +160,17 @@ struct NolbRecords {
 // NOLB records are laid out for ntiles tiles).  k_scan runs one tile per
 // workgroup; the segmented-op schedules (k_seg_sched) loop over tiles.
 
-template <class Op, int R, bool SEG, bool REDUCE, class In, int B, bool NOLB>
+template <class Op, int R, bool SEG, bool REDUCE, class In, int B, bool NOLB, int VB = 16>
 __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                           uint64_t *__restrict__ status, int64_t tile, int64_t ntiles,
                                           bool aligned, unsigned *__restrict__ err) {
     using T = typename Op::T;
-    using S = ScanShape<Op, R, B>;
+    using S = ScanShape<Op, R, B, VB>;
     constexpr int V = S::V;
+    static_assert(V <= 16, "end flags in bits 0..15 of eb, the word's earlier heads above");
     constexpr uint32_t VMASK = (1u << V) - 1u;
     static_assert(!NOLB || (SEG && REDUCE), "only segmented reductions run without look-back");
+    static_assert(VB == 16 || (SEG && REDUCE), "wide lanes only in segmented reductions");
     const Op op;
 
     __shared__ T s_wagg[S::WAVES];
@@ -396,7 +408,7 @@ __device__ __forceinline__ void scan_tile(const In &in, typename Op::T *__restri
 }
 
 template <class Op, int R, bool SEG, bool REDUCE, class In = ArrayIn<typename Op::T>, int B = kBlock,
-          bool NOLB = false>
+          bool NOLB = false, int VB = 16>
 __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                             uint64_t *__restrict__ status, unsigned *__restrict__ counter,
                                             int aligned_i, unsigned *__restrict__ err) {
@@ -405,19 +417,27 @@ __global__ __launch_bounds__(B) void k_scan(In in, typename Op::T *__restrict__ 
     // single-tile launch passes no counter
     if (threadIdx.x == 0) s_tile = counter ? atomicAdd(counter, 1u) : blockIdx.x;
     __syncthreads();
-    scan_tile<Op, R, SEG, REDUCE, In, B, NOLB>(in, out, n, sb, status, (int64_t)s_tile, (int64_t)gridDim.x,
-                                               aligned_i != 0, err);
+    scan_tile<Op, R, SEG, REDUCE, In, B, NOLB, VB>(in, out, n, sb, status, (int64_t)s_tile, (int64_t)gridDim.x,
+                                                   aligned_i != 0, err);
 }
 
-// Rows per wave and threads per workgroup of the tiles of a segmented
-// reduction (the CVL entries and generated fused reductions, so that both
-// give the same bits).  256 / 512 / 1024 threads: max_red of 2^26 doubles
-// 0.206 / 0.207 / 0.218 ms, the fused SpMV product (gathers in the loader)
-// 1.35 / 1.28 / 1.32 ms, fused quickhull 29.6 / 27.1 / 28.2 ms.
+// Rows per wave, threads per workgroup and bytes per lane and row of the
+// flat 8-row-arm tiles of a segmented reduction (the CVL entries and
+// generated fused reductions, so that both give the same bits); the
+// autotuner picks them (2 rows x 256 threads x 64 bytes: 4096 doubles or
+// 8192 ints per tile).
 #ifndef CVLHIP_SEG_REDUCE_ROWS
-#define CVLHIP_SEG_REDUCE_ROWS 8
+#define CVLHIP_SEG_REDUCE_ROWS 2
 #endif
 constexpr int kSegReduceRows = CVLHIP_SEG_REDUCE_ROWS;
+// Bytes each lane holds per row in those tiles: 64 (8 doubles, 16 ints) by
+// default.  A lane folds its consecutive elements in order before the wave
+// scan of the row, so the tile runs a quarter of the wave scans of 16-byte
+// lanes for the same elements (the scans were most of its instructions).
+#ifndef CVLHIP_SEG_REDUCE_LANE_BYTES
+#define CVLHIP_SEG_REDUCE_LANE_BYTES 64
+#endif
+constexpr int kSegReduceLaneBytes = CVLHIP_SEG_REDUCE_LANE_BYTES;
 // Rows per wave of the largest tiles of segmented scans (the flat8 arm).
 constexpr int kSegScanRows = 8;
 #ifndef CVLHIP_SEG_REDUCE_BLOCK

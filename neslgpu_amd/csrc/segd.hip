@@ -59,11 +59,19 @@ __device__ __forceinline__ void hist_add(unsigned *h, unsigned len) {
 // thread 0 stores the picks in the header.  Every thread of the workgroup
 // must call it.
 __device__ void sched_pick_block(SegdHeader *hdr, int64_t n, int64_t m, const unsigned *hist, unsigned *s_pick) {
+    // The network's parameters and the histogram are staged in LDS with
+    // one coalesced pass first, so the forward passes wait for memory once
+    // instead of once per layer.
+    __shared__ float s_P[kParams];
+    __shared__ unsigned s_h[kHist];
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) s_P[i] = g_sched_params[i];
+    if (threadIdx.x < kHist) s_h[threadIdx.x] = hist[threadIdx.x];
+    __syncthreads();
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (w < 4) {
         const bool red = (w >> 1) != 0, dbl = (w & 1) != 0;
-        const float f = sched_feature(lane < kF ? lane : 0, n, m, hist, hist + kLenClasses, dbl, red);
-        const int a = mlp_pick_wave(g_sched_params, f, sched_admitted(max_len_class(hist + kLenClasses), m));
+        const float f = sched_feature(lane < kF ? lane : 0, n, m, s_h, s_h + kLenClasses, dbl, red);
+        const int a = mlp_pick_wave(s_P, f, sched_admitted(max_len_class(s_h + kLenClasses), m));
         if (lane == 0) s_pick[w] = (unsigned)a;
     }
     __syncthreads();
@@ -318,8 +326,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_offsets(const int *__res
     }
 }
 
-// With `pick`, the workgroup of tile 0 also picks the schedules from the
-// histogram k_segd_offsets completed.
+// With `pick`, the launch has one workgroup more than tiles, and that last
+// workgroup picks the schedules from the histogram k_segd_offsets completed,
+// beside the tiles instead of after one of them.
 __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__restrict__ heads, int *wcnt,
                                                               int64_t words, uint64_t *status, unsigned *counter,
                                                               unsigned *err, SegdHeader *hdr, int64_t n, int64_t m,
@@ -329,6 +338,11 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__r
     __shared__ unsigned s_total;
     __shared__ unsigned s_pick[4];
     const int tid = threadIdx.x;
+    if (pick && blockIdx.x == gridDim.x - 1) {
+        sched_pick_block(hdr, n, m, hdr->hist, s_pick);
+        return;
+    }
+    const int64_t ntiles = (int64_t)gridDim.x - (pick ? 1 : 0);
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
@@ -347,8 +361,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_segd_wcnt(const uint32_t *__r
         const int64_t wd = w0 + k;
         if (wd < words) wcnt[wd] = (int)(P + loc[k]);
     }
-    if (tile == (int64_t)gridDim.x - 1 && tid == 0) wcnt[words] = (int)s_total;
-    if (pick && tile == 0) sched_pick_block(hdr, n, m, hdr->hist, s_pick);
+    if (tile == ntiles - 1 && tid == 0) wcnt[words] = (int)s_total;
 }
 
 __global__ void k_segd_empty(SegdHeader *hdr, int *offs, int64_t n) {
@@ -406,8 +419,8 @@ void segd_build(void *sd, const void *lens_in, int64_t n, int64_t m, void *scrat
         LAUNCHCHECK();
     }
     if (L.words > 0) {
-        k_segd_wcnt<<<(unsigned)tb, kSmallThreads, 0, g_stream>>>(heads, wcnt, L.words, st_b, counters + 1,
-                                                                  device_error_word(), hdr, n, m, pick);
+        k_segd_wcnt<<<(unsigned)(tb + (pick ? 1 : 0)), kSmallThreads, 0, g_stream>>>(
+            heads, wcnt, L.words, st_b, counters + 1, device_error_word(), hdr, n, m, pick);
         LAUNCHCHECK();
     } else {
         fill_u32(wcnt, 0, 1);

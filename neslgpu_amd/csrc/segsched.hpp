@@ -5,7 +5,10 @@
 // be scheduled several ways; all of them compute the serial semantics:
 //   arm 0  flat, 8 rows per wave   decoupled look-back (scans) or two-step
 //                                  records (reductions) over the flat vector;
-//                                  the fixed schedule of small vectors
+//                                  the fixed schedule of small vectors.
+//                                  Reductions: 2 rows of 64 bytes per lane
+//                                  (the same elements per tile, a quarter
+//                                  of the wave scans)
 //   arm 1  flat, 2 rows per wave   4x the tiles: more workgroups on mid-size
 //   arm 2  flat, 1 row per wave    vectors, more look-back per element
 //   arm 3  thread per segment      one lane walks a segment
@@ -48,9 +51,14 @@ __host__ __device__ constexpr int sched_block() {
     return REDUCE ? kSegReduceBlock : (sizeof(typename Op::T) == 4 ? CVLHIP_SEG_SCAN_BLOCK4 : 1024);
 }
 
-template <class Op, bool REDUCE, int R>
+template <class Op, bool REDUCE, int R, int VB = 16>
 __host__ __device__ constexpr int64_t sched_tile() {
-    return ScanShape<Op, R, sched_block<Op, REDUCE>()>::TILE;
+    return ScanShape<Op, R, sched_block<Op, REDUCE>(), VB>::TILE;
+}
+// The 8-row arm's tile: rows of wide lanes for reductions.
+template <class Op, bool REDUCE>
+__host__ __device__ constexpr int64_t sched_tile8() {
+    return REDUCE ? sched_tile<Op, true, kSegReduceRows, kSegReduceLaneBytes>() : sched_tile<Op, false, kSegScanRows>();
 }
 
 // Scratch: a 256-byte header (the tile counter of scans at byte 0, the tile
@@ -180,22 +188,23 @@ __device__ __forceinline__ void sched_block_arm(const In &in, typename Op::T *__
 // as many workgroups as it has tiles).  The finer arms loop over tiles when
 // the grid has fewer workgroups than they have tiles; a loop around the
 // 8-row tile would cost it a third more registers and an occupancy step.
-template <class Op, int R, bool REDUCE, class In, int B, bool LOOP>
+template <class Op, int R, bool REDUCE, class In, int B, bool LOOP, int VB = 16>
 __device__ __forceinline__ void sched_flat(const In &in, typename Op::T *__restrict__ out, int64_t n, SegBits sb,
                                            void *scratch, bool aligned, unsigned *err) {
-    constexpr int64_t TILE = ScanShape<Op, R, B>::TILE;
+    constexpr int64_t TILE = ScanShape<Op, R, B, VB>::TILE;
     const int64_t tiles = (n + TILE - 1) / TILE;
     uint64_t *status = (uint64_t *)((char *)scratch + kSchedHeader);
     if constexpr (REDUCE) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *(int64_t *)((char *)scratch + 8) = tiles;
         if constexpr (LOOP) {
             for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-                scan_tile<Op, R, true, true, In, B, true>(in, out, n, sb, status, t, tiles, aligned, err);
+                scan_tile<Op, R, true, true, In, B, true, VB>(in, out, n, sb, status, t, tiles, aligned, err);
                 __syncthreads();
             }
         } else {
             const int64_t t = blockIdx.x;
-            if (t < tiles) scan_tile<Op, R, true, true, In, B, true>(in, out, n, sb, status, t, tiles, aligned, err);
+            if (t < tiles)
+                scan_tile<Op, R, true, true, In, B, true, VB>(in, out, n, sb, status, t, tiles, aligned, err);
         }
     } else {
         __shared__ unsigned s_t;
@@ -246,8 +255,8 @@ __global__ __launch_bounds__((sched_block<Op, REDUCE>())) void k_seg_sched(In in
         else sched_block_arm<Op, REDUCE, In, B>(in, out, sv);
         return;
     default:
-        sched_flat<Op, REDUCE ? kSegReduceRows : kSegScanRows, REDUCE, In, B, false>(in, out, n, sb, scratch, aligned,
-                                                                                   err);
+        sched_flat<Op, REDUCE ? kSegReduceRows : kSegScanRows, REDUCE, In, B, false,
+                   REDUCE ? kSegReduceLaneBytes : 16>(in, out, n, sb, scratch, aligned, err);
         return;
     }
 }
@@ -275,8 +284,7 @@ inline int sched_grid(int64_t n) {
                                                            sched_block<Op, REDUCE>(), 0);
         resident = (cus > 0 ? cus : 1) * (per > 0 ? per : 1);
     }
-    constexpr int64_t tile1 = sched_tile<Op, REDUCE, 1>(),
-                      tile8 = sched_tile<Op, REDUCE, REDUCE ? kSegReduceRows : kSegScanRows>();
+    constexpr int64_t tile1 = sched_tile<Op, REDUCE, 1>(), tile8 = sched_tile8<Op, REDUCE>();
     const int64_t t1 = (n + tile1 - 1) / tile1, t8 = (n + tile8 - 1) / tile8;
     int64_t g = t1 < resident ? t1 : resident;
     if (g < t8) g = t8;
@@ -294,12 +302,13 @@ inline void seg_reduce_run(const In &in, typename Op::T *out, int64_t n, const S
     using T = typename Op::T;
     if (n <= 0) return;
     if (arm < 0 && n < kSchedMinN) {
-        using S = ScanShape<Op, kSegReduceRows, kSegReduceBlock>;
+        using S = ScanShape<Op, kSegReduceRows, kSegReduceBlock, kSegReduceLaneBytes>;
         const int64_t tiles = (n + S::TILE - 1) / S::TILE;
         void *recs = (char *)scratch + kSchedHeader;
         const SegBits sb = from_hdr ? SegBits{nullptr, nullptr, nullptr, sd, 0}
                                     : SegBits{sv.heads, sv.wcnt, sv.nz, nullptr, sv.m};
-        k_scan<Op, kSegReduceRows, true, true, In, kSegReduceBlock, true><<<(unsigned)tiles, kSegReduceBlock, 0, st>>>(
+        k_scan<Op, kSegReduceRows, true, true, In, kSegReduceBlock, true, kSegReduceLaneBytes>
+            <<<(unsigned)tiles, kSegReduceBlock, 0, st>>>(
             in, out, n, sb, (uint64_t *)recs, nullptr, aligned, err);
         if (tiles > 1) k_seg_reduce_fix<Op><<<1, kFixThreads, 0, st>>>(NolbRecords<T>::at(recs, tiles), tiles, out);
         return;

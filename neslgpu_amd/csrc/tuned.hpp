@@ -43,10 +43,13 @@
 #define CVLHIP_RANK_CHAIN 0
 #endif
 #ifndef CVLHIP_SEG_REDUCE_ROWS
-#define CVLHIP_SEG_REDUCE_ROWS 8
+#define CVLHIP_SEG_REDUCE_ROWS 2
 #endif
 #ifndef CVLHIP_SEG_REDUCE_BLOCK
 #define CVLHIP_SEG_REDUCE_BLOCK 256
+#endif
+#ifndef CVLHIP_SEG_REDUCE_LANE_BYTES
+#define CVLHIP_SEG_REDUCE_LANE_BYTES 64
 #endif
 #ifndef CVLHIP_PAIR_ROWS
 #define CVLHIP_PAIR_ROWS 8

@@ -55,8 +55,9 @@ SPACE = {
     "CVLHIP_RANK_THREADS": ([256, 512, 1024], "rku_luz"),
     "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
-    "CVLHIP_SEG_REDUCE_ROWS,CVLHIP_SEG_REDUCE_BLOCK": ([(8, 256), (4, 256), (6, 256), (4, 512), (6, 512),
-                                                       (8, 512), (12, 256)], "max_red"),
+    "CVLHIP_SEG_REDUCE_ROWS,CVLHIP_SEG_REDUCE_BLOCK,CVLHIP_SEG_REDUCE_LANE_BYTES": (
+        [(2, 256, 64), (8, 256, 16), (4, 256, 32), (4, 256, 64), (1, 256, 64), (2, 512, 64), (3, 256, 64)],
+        "max_red"),
     # a comma-joined key searches its macros together (values are tuples)
     "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(4, 256), (8, 256), (4, 512), (8, 512), (4, 1024)], "add_sed_max_red"),
 }
