@@ -23,12 +23,20 @@ namespace cvlhip {
 #ifndef CVLHIP_PAIR_BLOCK
 #define CVLHIP_PAIR_BLOCK 512
 #endif
+// Bytes each lane holds per row (one 16-byte load by default).  Wider lanes
+// fold their consecutive doubles in order before the row's wave scans, as
+// the reduction tiles do, but here every element is also stored: 64-byte
+// lanes ran 0.415 ms against 0.362 (2^26 doubles, Zipf(1.5)).
+#ifndef CVLHIP_PAIR_LANE_BYTES
+#define CVLHIP_PAIR_LANE_BYTES 16
+#endif
 constexpr int kPairRows = CVLHIP_PAIR_ROWS;
 constexpr int kPairBlock = CVLHIP_PAIR_BLOCK;
 
 template <int R, int B>
 struct PairShape {
-    static constexpr int V = 2;                      // doubles per lane and row (one 16-byte load)
+    static constexpr int V = CVLHIP_PAIR_LANE_BYTES / 8;   // doubles per lane and row
+    static_assert(V % 2 == 0 && V <= 16, "whole 16-byte loads; end flags in bits 0..15");
     static constexpr int ROW = kWave * V;
     static constexpr int WAVES = B / kWave;
     static constexpr int WAVE_ELEMS = R * ROW;

@@ -64,11 +64,16 @@ __device__ __forceinline__ void load_lane(const typename Op::T *__restrict__ in,
 template <class T, int V>
 __device__ __forceinline__ void store_lane(T *__restrict__ out, int64_t base, int64_t n, bool aligned,
                                            const T (&y)[V]) {
+    constexpr int E = 16 / (int)sizeof(T);          // elements per 16-byte store
+    static_assert(V % E == 0, "whole 16-byte stores per lane");
     if (aligned && base + V <= n) {
-        typename Vec16<T>::type v;
 #pragma unroll
-        for (int k = 0; k < V; k++) v[k] = y[k];
-        *(typename Vec16<T>::type *)(out + base) = v;
+        for (int q = 0; q < V / E; q++) {
+            typename Vec16<T>::type v;
+#pragma unroll
+            for (int k = 0; k < E; k++) v[k] = y[q * E + k];
+            *(typename Vec16<T>::type *)(out + base + q * E) = v;
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < V; k++)

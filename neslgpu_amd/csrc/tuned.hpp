@@ -57,3 +57,6 @@
 #ifndef CVLHIP_PAIR_BLOCK
 #define CVLHIP_PAIR_BLOCK 512
 #endif
+#ifndef CVLHIP_PAIR_LANE_BYTES
+#define CVLHIP_PAIR_LANE_BYTES 16
+#endif

@@ -59,7 +59,9 @@ SPACE = {
         [(2, 256, 64), (8, 256, 16), (4, 256, 32), (4, 256, 64), (1, 256, 64), (2, 512, 64), (3, 256, 64)],
         "max_red"),
     # a comma-joined key searches its macros together (values are tuples)
-    "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK": ([(4, 256), (8, 256), (4, 512), (8, 512), (4, 1024)], "add_sed_max_red"),
+    "CVLHIP_PAIR_ROWS,CVLHIP_PAIR_BLOCK,CVLHIP_PAIR_LANE_BYTES": (
+        [(8, 512, 16), (4, 512, 32), (2, 512, 64), (2, 256, 64), (4, 256, 64), (1, 512, 64), (2, 1024, 64)],
+        "add_sed_max_red"),
 }
 
 
