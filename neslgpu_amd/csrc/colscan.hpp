@@ -20,7 +20,10 @@
 
 namespace cvlhip {
 
-constexpr int kTileScanRows = 64;
+#ifndef CVLHIP_TILE_SCAN_ROWS
+#define CVLHIP_TILE_SCAN_ROWS 64
+#endif
+constexpr int kTileScanRows = CVLHIP_TILE_SCAN_ROWS;   // rows (tiles) per workgroup
 constexpr int kTileScanCols = 256;
 
 __global__ __launch_bounds__(kTileScanCols) inline void k_tile_scan(unsigned *__restrict__ m, int64_t tiles,

@@ -42,6 +42,9 @@
 #ifndef CVLHIP_RANK_CHAIN
 #define CVLHIP_RANK_CHAIN 0
 #endif
+#ifndef CVLHIP_TILE_SCAN_ROWS
+#define CVLHIP_TILE_SCAN_ROWS 64
+#endif
 #ifndef CVLHIP_SEG_REDUCE_ROWS
 #define CVLHIP_SEG_REDUCE_ROWS 2
 #endif

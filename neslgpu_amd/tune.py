@@ -55,6 +55,7 @@ SPACE = {
     "CVLHIP_RANK_THREADS": ([256, 512, 1024], "rku_luz"),
     "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
+    "CVLHIP_TILE_SCAN_ROWS": ([64, 32, 128, 16], "rku_luz"),
     "CVLHIP_SEG_REDUCE_ROWS,CVLHIP_SEG_REDUCE_BLOCK,CVLHIP_SEG_REDUCE_LANE_BYTES": (
         [(2, 256, 64), (8, 256, 16), (4, 256, 32), (4, 256, 64), (1, 256, 64), (2, 512, 64), (3, 256, 64)],
         "max_red"),
