@@ -58,6 +58,8 @@ def shapes():
         ("small", split(5000, 100, 0.1)),
         ("small_edge", split(4096 * 32, 4096, 0.1)),     # the largest one-workgroup descriptor
         ("few_long", split(500_000, 3000, 0.05)),        # look-back kernels: bitmap above 4096 words
+        ("mid_tiles", split(700_000, 20_000, 0.2)),      # several tiles of lengths and of words, empties
+        ("mid_edge", split(1 << 20, 1 << 15, 0.1)),      # 8 tiles of each, exactly
         ("many_short", split(120_000, 50_000, 0.2)),     # more than 4096 lengths
         ("large_long", split(1 << 21, 70_000)),
         ("large_many", split(300_000, 200_000, 0.3)),
