@@ -410,14 +410,21 @@ template <class KeyT, int KOUT, bool FUSE = false>
 __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     static_assert(!FUSE || KOUT == 0, "the fused last pass writes no keys");
     constexpr int W = kRankThreads / kWave;
-    // keys, then (reinterpreted) indices, in digit order; or (FUSE) the
-    // 8-byte pairs in child order
-    constexpr int kStage = FUSE ? kRankTile * 8 / (int)sizeof(KeyT) : kRankTile;
-    __shared__ unsigned s_cnt[W][kRadix];     // per-wave digit counts, then per-wave starts within a digit
+    // s_cnt: per-wave digit counts, then per-wave starts within a digit;
+    // s_stage: keys, then (reinterpreted) indices, in digit order.  FUSE:
+    // the 8-byte pairs in child order, staged once every position is known,
+    // over both (then dead): 32 KiB instead of 8 + 32, so three workgroups
+    // of LDS become four.
+    constexpr int kCntBytes = W * kRadix * 4;
+    constexpr int kStageBytes = kRankTile * (int)sizeof(KeyT);
+    constexpr int kPairBytes = kRankTile * 8;
+    constexpr int kRaw = (FUSE && kPairBytes > kCntBytes + kStageBytes) ? kPairBytes : kCntBytes + kStageBytes;
+    __shared__ __attribute__((aligned(16))) unsigned char s_raw[kRaw];
+    unsigned(*const s_cnt)[kRadix] = (unsigned(*)[kRadix])s_raw;
+    KeyT *const s_stage = (KeyT *)(s_raw + kCntBytes);
     __shared__ unsigned s_tot[kRadix];
     __shared__ unsigned s_lstart[kRadix];     // tile-local start of each digit's run
     __shared__ unsigned s_gbase[kRadix];      // global position of the run
-    __shared__ KeyT s_stage[kStage];
     __shared__ unsigned char s_dig[FUSE ? 1 : kRankTile];
     __shared__ unsigned s_c2[FUSE ? kRadix : 1];   // FUSE: per-child counts, then starts, of the tile
     __shared__ unsigned s_g2[FUSE ? kRadix : 1];   // FUSE: the tile's slot in each child
@@ -569,7 +576,7 @@ loaded:
         // (index, position) in child index >> clog of the inverse's first
         // partition pass, staged in child order, each child's run written
         // contiguously at the tile's slot in that child.
-        uint2 *s_pair = (uint2 *)s_stage;
+        uint2 *const s_pair = (uint2 *)s_raw;   // written only after the barrier below: s_cnt is read here
         unsigned ch[kRankItems], rk[kRankItems], pos[kRankItems];
 #pragma unroll
         for (int j = 0; j < kRankItems; j++) {
