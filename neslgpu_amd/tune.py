@@ -52,8 +52,8 @@ SPACE = {
     "CVLHIP_PART_THREADS": ([512, 1024], "smp_puz"),
     "CVLHIP_BLK_LOG4": ([12, 13, 14], "smp_puz"),
     "CVLHIP_PART_ATOMIC_FIRST": ([0, 1], "smp_puz"),
-    "CVLHIP_RANK_THREADS": ([256, 512, 1024], "rku_luz"),
-    "CVLHIP_RANK_ITEMS": ([4, 8], "rku_luz"),
+    "CVLHIP_RANK_THREADS,CVLHIP_RANK_ITEMS": ([(512, 8), (256, 16), (512, 16), (1024, 8), (256, 8), (1024, 4)],
+                                              "rku_luz"),
     "CVLHIP_RANK_CHAIN": ([0, 1], "rku_luz"),
     "CVLHIP_TILE_SCAN_ROWS": ([64, 32, 128, 16], "rku_luz"),
     "CVLHIP_SEG_REDUCE_ROWS,CVLHIP_SEG_REDUCE_BLOCK,CVLHIP_SEG_REDUCE_LANE_BYTES": (
