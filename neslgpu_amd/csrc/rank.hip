@@ -449,7 +449,6 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
     int idx[kRankItems];
     unsigned dig[kRankItems];
     unsigned loc[kRankItems];
-    const uint64_t lt = (lane == 0) ? 0 : (~0ull >> (64 - lane));
 
     // Item j of a lane is element wb + j*64 + lane: each round of the
     // multisplit below holds 64 consecutive elements, which keeps it stable.
@@ -503,38 +502,51 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_pass(PassArgs a) {
         idx[j] = ix;
     }
 loaded:
+    // FULL: every element of the wave's span exists (all but the last
+    // wave of the vector), so no lane is masked out.
+    auto multisplit = [&](auto full) {
+        constexpr bool FULL = decltype(full)::value;
 #pragma unroll
-    for (int j = 0; j < kRankItems; j++) {
-        const int64_t e = wb + (int64_t)j * kWave + lane;
-        const bool in = e < a.n;
-        const KeyT k = key[j];
-        const unsigned d = in ? (unsigned)((k >> a.shift) & 255) : 0u;
-        dig[j] = d;
-        // lanes of this round holding the same digit (inactive lanes
-        // excluded): the lanes that differ from this one in some bit of the
-        // digit are the OR over the bits of (lanes with the bit set) XOR
-        // (all lanes if this lane has it set, none if not)
-        // Per bit four instructions: the sign-extended bit, its lane mask
-        // (a compare of that value, which the compiler would otherwise
-        // rebuild from a shift of the digit), and one 3-input bit operation
-        // (v_bitop3, gfx950) per half of the mask: acc | (spread ^ mask),
-        // truth table 0xde over (spread, acc, mask).
-        uint32_t dlo = 0, dhi = 0;
+        for (int j = 0; j < kRankItems; j++) {
+            const bool in = FULL || wb + (int64_t)j * kWave + lane < a.n;
+            const KeyT k = key[j];
+            const unsigned d = in ? (unsigned)((k >> a.shift) & 255) : 0u;
+            dig[j] = d;
+            // lanes of this round holding the same digit: those that agree
+            // with this lane in every bit of the digit, the AND over the
+            // bits of (lanes with the bit set) XNOR (all lanes if this lane
+            // has it set, none if not).  Per bit four instructions: the
+            // sign-extended bit, its lane mask (a compare of that value,
+            // which the compiler would otherwise rebuild from a shift of the
+            // digit), and one 3-input bit operation (v_bitop3, gfx950) per
+            // half of the mask: acc & ~(spread ^ mask), truth table 0x84
+            // over (spread, acc, mask).
+            uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const uint32_t spread = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);   // 0 or ~0: bit b
-            const uint64_t bb = __builtin_amdgcn_uicmp(spread, 0u, 33);            // lanes with bit b (ICMP_NE)
-            dlo = __builtin_amdgcn_bitop3_b32(spread, dlo, (uint32_t)bb, 0xde);
-            dhi = __builtin_amdgcn_bitop3_b32(spread, dhi, (uint32_t)(bb >> 32), 0xde);
+            for (int b = 0; b < 8; b++) {
+                const uint32_t spread = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);   // 0 or ~0: bit b
+                const uint64_t bb = __builtin_amdgcn_uicmp(spread, 0u, 33);            // lanes with bit b (ICMP_NE)
+                plo = __builtin_amdgcn_bitop3_b32(spread, plo, (uint32_t)bb, 0x84);
+                phi = __builtin_amdgcn_bitop3_b32(spread, phi, (uint32_t)(bb >> 32), 0x84);
+            }
+            if constexpr (!FULL) {             // lanes past the end are no one's peers
+                const uint64_t act = ballot(in);
+                plo &= (uint32_t)act;
+                phi &= (uint32_t)(act >> 32);
+            }
+            // peers in lanes below this one (v_mbcnt: the set bits of a
+            // 64-lane mask below the current lane)
+            const unsigned before = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+            const uint64_t peers = ((uint64_t)phi << 32) | plo;
+            const unsigned old = s_cnt[w][d];
+            loc[j] = old + before;
+            // the highest lane of each digit group advances the counter
+            const bool leader = in && (peers >> lane) == 1ull;
+            if (leader) s_cnt[w][d] = old + before + 1;
         }
-        const uint64_t peers = ~(((uint64_t)dhi << 32) | dlo) & ballot(in);
-        const unsigned before = (unsigned)__popcll(peers & lt);
-        const unsigned old = s_cnt[w][d];
-        loc[j] = old + before;
-        // the highest lane of each digit group advances the counter
-        const bool leader = in && (peers >> lane) == 1ull;
-        if (leader) s_cnt[w][d] = old + before + 1;
-    }
+    };
+    if (wb + kWaveSpan <= a.n) multisplit(std::true_type{});
+    else multisplit(std::false_type{});
     __syncthreads();
 
     // Per digit: wave starts within the digit, the tile count, and the
