@@ -42,13 +42,18 @@ enum : int {
 };
 
 // Threads per workgroup: those of the fixed schedule of each kind
-// (segmented 4-byte scans 512, 8-byte scans 1024, reductions 512).
+// (segmented 4-byte scans CVLHIP_SEG_SCAN_BLOCK4, 8-byte scans
+// CVLHIP_SEG_SCAN_BLOCK8, reductions CVLHIP_SEG_REDUCE_BLOCK).
 #ifndef CVLHIP_SEG_SCAN_BLOCK4
 #define CVLHIP_SEG_SCAN_BLOCK4 512
 #endif
+#ifndef CVLHIP_SEG_SCAN_BLOCK8
+#define CVLHIP_SEG_SCAN_BLOCK8 1024
+#endif
 template <class Op, bool REDUCE>
 __host__ __device__ constexpr int sched_block() {
-    return REDUCE ? kSegReduceBlock : (sizeof(typename Op::T) == 4 ? CVLHIP_SEG_SCAN_BLOCK4 : 1024);
+    return REDUCE ? kSegReduceBlock
+                  : (sizeof(typename Op::T) == 4 ? CVLHIP_SEG_SCAN_BLOCK4 : CVLHIP_SEG_SCAN_BLOCK8);
 }
 
 template <class Op, bool REDUCE, int R, int VB = 16>

@@ -15,6 +15,9 @@
 #ifndef CVLHIP_SEG_SCAN_BLOCK4
 #define CVLHIP_SEG_SCAN_BLOCK4 512
 #endif
+#ifndef CVLHIP_SEG_SCAN_BLOCK8
+#define CVLHIP_SEG_SCAN_BLOCK8 1024
+#endif
 #ifndef CVLHIP_REDUCE_GRID
 #define CVLHIP_REDUCE_GRID 2048
 #endif

@@ -46,6 +46,7 @@ SPACE = {
                                                  "add_suz"),
     "CVLHIP_LOOKBACK_K": ([1, 2, 4], "add_suz"),
     "CVLHIP_SEG_SCAN_BLOCK4": ([256, 512, 1024], "add_sez"),
+    "CVLHIP_SEG_SCAN_BLOCK8": ([1024, 512, 256], "add_sed"),
     "CVLHIP_REDUCE_GRID": ([1024, 2048], "max_ruz"),
     "CVLHIP_PACK_ROWS": ([4, 6, 8, 10], "pk2_luz"),
     "CVLHIP_PART_ITEMS": ([4, 8, 12], "smp_puz"),
