@@ -13,8 +13,8 @@ dispatch_model.npz), and stores the pick in the descriptor; every CVL
 segmented call on that descriptor runs the picked arm, with no host round
 trip.  The network's forward and backward passes and its Adam steps are HIP
 kernels (cvlhip_mlp_*); this module only orchestrates them.  Only the
-schedule changes with the choice, never the semantics; the autotuner stays
-the judge of the dispatcher's regret.
+schedule changes with the choice, never the semantics; the autotuner's
+exhaustive timings stay the measure of the dispatcher's regret.
 
 Three workload sets, kept apart:
   TRAIN  -- timings the network is fitted to;
