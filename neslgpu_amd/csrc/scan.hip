@@ -39,12 +39,20 @@ __global__ __launch_bounds__(kWave) void k_reduce_finish(const typename Op::T *_
     using T = typename Op::T;
     const Op op;
     const int lane = threadIdx.x;
-    T tot = Op::neutral();
-    for (int i0 = 0; i0 < g; i0 += kWave) {
-        const int i = i0 + lane;
-        const T v = i < g ? partials[i] : Op::neutral();
-        tot = op(tot, wave_reduce(v, op));
+    // Every partial is loaded before the fold (they were written on other
+    // XCDs: one memory latency instead of one per 64), then folded 64 at a
+    // time in block order.
+    constexpr int K = kReducePartials / kWave;
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = k * kWave + lane;
+        v[k] = i < g ? partials[i] : Op::neutral();
     }
+    T tot = Op::neutral();
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (k * kWave < g) tot = op(tot, wave_reduce(v[k], op));
     if (lane == 0) {
         const T r = op(Op::cvl_id(), tot);
         result[0] = r;
