@@ -48,23 +48,42 @@ __global__ __launch_bounds__(kTileScanCols) inline void k_tile_scan(unsigned *__
         st_status(mine, ((uint64_t)kPrefix << 32) | sum);
     } else {
         st_status(mine, ((uint64_t)kAggregate << 32) | sum);
+        // Walk back over the earlier groups, LB status words loaded at once
+        // (independent loads: one latency per LB groups) and taken in order
+        // up to the first prefix; a group not yet published stops the batch
+        // and the walk resumes there.  Group 0 always publishes a prefix,
+        // so the walk ends at t = 0 at the latest.
+        constexpr int LB = 8;
         int64_t t = g - 1;
         int spins = 0;
         for (;;) {
-            const uint64_t w = ld_status(status + t * kTileScanCols + d);
-            const uint32_t f = (uint32_t)(w >> 32);
-            if (f == kInvalid) {
+            uint64_t w[LB];
+#pragma unroll
+            for (int k = 0; k < LB; k++) w[k] = t - k >= 0 ? ld_status(status + (t - k) * kTileScanCols + d) : 0;
+            int used = 0;          // words taken from this batch
+            bool done = false, wait = false;
+#pragma unroll
+            for (int k = 0; k < LB; k++) {
+                if (done || wait) continue;
+                const uint32_t f = (uint32_t)(w[k] >> 32);
+                if (f == kInvalid) {
+                    wait = true;
+                    continue;
+                }
+                excl += (uint32_t)w[k];
+                used = k + 1;
+                if (f == kPrefix) done = true;
+            }
+            if (done) break;
+            t -= used;
+            if (used > 0) spins = 0;
+            if (wait) {
                 if (++spins >= kSpinLimit) {
                     atomicOr(err, 2u);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            excl += (uint32_t)w;
-            if (f == kPrefix) break;
-            t--;
-            spins = 0;
         }
         st_status(mine, ((uint64_t)kPrefix << 32) | (excl + sum));
     }
